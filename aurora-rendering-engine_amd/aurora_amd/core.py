@@ -1,0 +1,285 @@
+"""Python handles over libaurora's C ABI.
+
+`SceneArrays` is the flattened scene (structure-of-arrays numpy buffers, the
+layout of aurora_scene_desc).  `Context` owns a device and stream; `Scene`
+owns an uploaded scene and its LBVH.  Errors from the library raise
+ValueError (AURORA_ERR_INVALID) or RuntimeError (everything else), the
+Python analogues of the std::invalid_argument / std::runtime_error the
+reference's constructors throw (src/object/triangle.cpp:13-36,
+src/texture.cpp:11-43).
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class SceneArrays:
+    spheres: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.float32))
+    sphere_mat: np.ndarray = field(default_factory=lambda: np.zeros((0,), np.uint32))
+    quads: np.ndarray = field(default_factory=lambda: np.zeros((0, 9), np.float32))
+    quad_mat: np.ndarray = field(default_factory=lambda: np.zeros((0,), np.uint32))
+    quad_kind: np.ndarray = field(default_factory=lambda: np.zeros((0,), np.uint32))
+    materials: list = field(default_factory=list)  # (type, texture, fuzz, ior)
+    textures: list = field(default_factory=list)   # dicts, see texture()
+    texels: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.float32))
+
+    @property
+    def n_prims(self):
+        return len(self.spheres) + len(self.quads)
+
+    def desc(self):
+        """Build an aurora_scene_desc; returns (desc, keepalive)."""
+        sp = np.ascontiguousarray(self.spheres, np.float32).reshape(-1, 4)
+        sm = np.ascontiguousarray(self.sphere_mat, np.uint32).reshape(-1)
+        qd = np.ascontiguousarray(self.quads, np.float32).reshape(-1, 9)
+        qm = np.ascontiguousarray(self.quad_mat, np.uint32).reshape(-1)
+        qk = np.ascontiguousarray(self.quad_kind, np.uint32).reshape(-1)
+        if len(qk) != len(qd):
+            qk = np.zeros(len(qd), np.uint32)
+        mats = (abi.Material * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            mats[i] = abi.Material(int(m[0]), int(m[1]), float(m[2]), float(m[3]))
+        texs = (abi.Texture * max(1, len(self.textures)))()
+        for i, t in enumerate(self.textures):
+            texs[i] = abi.Texture(
+                int(t.get("type", 0)), int(t.get("even", 0)), int(t.get("odd", 0)), int(t.get("seed", 0)),
+                (C.c_float * 3)(*[float(x) for x in t.get("color", (0, 0, 0))]), float(t.get("scale", 1.0)),
+                int(t.get("width", 0)), int(t.get("height", 0)), int(t.get("texel_offset", 0)))
+        tx = np.ascontiguousarray(self.texels, np.float32).reshape(-1, 3)
+        d = abi.SceneDesc()
+        d.n_spheres = len(sp)
+        d.spheres = sp.ctypes.data if len(sp) else None
+        d.sphere_material = sm.ctypes.data if len(sm) else None
+        d.n_quads = len(qd)
+        d.quads = qd.ctypes.data if len(qd) else None
+        d.quad_material = qm.ctypes.data if len(qm) else None
+        d.quad_kind = qk.ctypes.data if len(qk) else None
+        d.n_materials = len(self.materials)
+        d.materials = C.addressof(mats)
+        d.n_textures = len(self.textures)
+        d.textures = C.addressof(texs)
+        d.n_texels = len(tx)
+        d.texels = tx.ctypes.data if len(tx) else None
+        return d, (sp, sm, qd, qm, qk, mats, texs, tx)
+
+
+def solid(r, g, b):
+    return {"type": abi.TEX_SOLID, "color": (r, g, b)}
+
+
+def checker(scale, even, odd):
+    return {"type": abi.TEX_CHECKER, "scale": scale, "even": even, "odd": odd}
+
+
+def noise(scale, seed=0):
+    return {"type": abi.TEX_NOISE, "scale": scale, "seed": seed}
+
+
+def image(width, height, texel_offset):
+    return {"type": abi.TEX_IMAGE, "width": width, "height": height, "texel_offset": texel_offset}
+
+
+@dataclass
+class Camera:
+    image_width: int
+    image_height: int
+    vfov: float
+    lookfrom: tuple
+    lookat: tuple
+    vup: tuple = (0.0, 1.0, 0.0)
+    defocus_angle: float = 0.0
+    focus_dist: float = 10.0
+
+    def desc(self):
+        c = abi.CameraDesc()
+        c.image_width = self.image_width
+        c.image_height = self.image_height
+        c.vfov = self.vfov
+        c.lookfrom = (C.c_float * 3)(*self.lookfrom)
+        c.lookat = (C.c_float * 3)(*self.lookat)
+        c.vup = (C.c_float * 3)(*self.vup)
+        c.defocus_angle = self.defocus_angle
+        c.focus_dist = self.focus_dist
+        return c
+
+
+@dataclass
+class RenderSettings:
+    spp: int
+    max_depth: int = 50
+    seed: int = 0
+    sample_offset: int = 0
+    background: tuple = (0.0, 0.0, 0.0)
+    aov: bool = False
+    count_fetches: bool = False
+    spp_per_pass: int = 0
+    tmin: float = 0.0
+
+    def desc(self):
+        r = abi.RenderDesc()
+        r.spp = self.spp
+        r.sample_offset = self.sample_offset
+        r.max_depth = self.max_depth
+        r.seed = self.seed
+        r.background = (C.c_float * 3)(*self.background)
+        r.flags = (abi.RENDER_AOV if self.aov else 0) | (abi.RENDER_COUNT_FETCHES if self.count_fetches else 0)
+        r.spp_per_pass = self.spp_per_pass
+        r.tmin = self.tmin
+        return r
+
+
+def stats_dict(st):
+    return {k: getattr(st, k) if not isinstance(getattr(st, k), C.Array) else list(getattr(st, k))
+            for k, _ in st._fields_}
+
+
+class Context:
+    def __init__(self, device=0):
+        self.lib = abi.load_library()
+        h = C.c_void_p()
+        rc = self.lib.aurora_create(device, C.byref(h))
+        if rc != abi.AURORA_OK:
+            raise RuntimeError(f"aurora_create failed ({rc}): {self.lib.aurora_last_error(None).decode()}")
+        self.h = h
+
+    def check(self, rc):
+        if rc != abi.AURORA_OK:
+            msg = self.lib.aurora_last_error(self.h).decode()
+            raise (ValueError if rc == abi.AURORA_ERR_INVALID else RuntimeError)(f"libaurora error {rc}: {msg}")
+
+    def close(self):
+        if self.h:
+            self.lib.aurora_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_launch(self, extend_blocks_per_cu, shade_blocks_per_cu):
+        self.check(self.lib.aurora_set_launch(self.h, extend_blocks_per_cu, shade_blocks_per_cu))
+
+    def synchronize(self):
+        self.check(self.lib.aurora_synchronize(self.h))
+
+    def stream_handle(self):
+        return self.lib.aurora_stream(self.h)
+
+    def upload(self, arrays: SceneArrays):
+        return Scene(self, arrays)
+
+
+class Scene:
+    def __init__(self, ctx: Context, arrays: SceneArrays):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.arrays = arrays
+        d, keep = arrays.desc()
+        h = C.c_void_p()
+        ctx.check(self.lib.aurora_scene_upload(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.bvh_stats = None
+
+    def close(self):
+        if self.h:
+            self.lib.aurora_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build_bvh(self):
+        st = abi.BvhStats()
+        self.ctx.check(self.lib.aurora_bvh_build(self.h, C.byref(st)))
+        self.bvh_stats = stats_dict(st)
+        return self.bvh_stats
+
+    def nodes(self):
+        n = self.bvh_stats["n_nodes"]
+        buf = np.zeros((max(n, 1), 8), np.uint32)
+        self.ctx.check(self.lib.aurora_bvh_download(self.h, buf.ctypes.data, n))
+        return buf[:n]
+
+    def order(self):
+        n = self.arrays.n_prims
+        m = np.zeros(max(n, 1), np.uint32)
+        o = np.zeros(max(n, 1), np.uint32)
+        self.ctx.check(self.lib.aurora_bvh_download_order(self.h, m.ctypes.data, o.ctypes.data, n))
+        return m[:n], o[:n]
+
+    def trace(self, origins, dirs, tmin=0.0, tmax=np.inf):
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = len(o)
+        prim = np.zeros(max(n, 1), np.int32)
+        t = np.zeros(max(n, 1), np.float32)
+        self.ctx.check(self.lib.aurora_trace_closest(self.h, o.ctypes.data, d.ctypes.data, n, tmin, tmax,
+                                                     prim.ctypes.data, t.ctypes.data))
+        return prim[:n], t[:n]
+
+    def trace_device(self, o, d, prim, t, tmin=0.0, tmax=float("inf"), counts=None, brute=False):
+        """o, d: device tensors (n,3) float32; prim int32 (n,), t float32 (n,);
+        counts optional uint32/int32 (n,3).  Asynchronous on the context stream."""
+        n = o.shape[0]
+        if brute:
+            rc = self.lib.aurora_trace_brute_device(self.h, o.data_ptr(), d.data_ptr(), n, tmin, tmax,
+                                                    prim.data_ptr(), t.data_ptr())
+        else:
+            rc = self.lib.aurora_trace_closest_device(self.h, o.data_ptr(), d.data_ptr(), n, tmin, tmax,
+                                                      prim.data_ptr(), t.data_ptr(),
+                                                      counts.data_ptr() if counts is not None else None)
+        self.ctx.check(rc)
+
+    def render(self, cam: Camera, rs: RenderSettings, rgb=None, albedo=None, normal=None):
+        """Host-buffer render; returns (rgb_sum, albedo_sum, normal_sum, stats)."""
+        W, H = cam.image_width, cam.image_height
+        rgb = np.zeros((H, W, 3), np.float32) if rgb is None else np.ascontiguousarray(rgb, np.float32)
+        if rs.aov:
+            albedo = np.zeros((H, W, 3), np.float32) if albedo is None else np.ascontiguousarray(albedo, np.float32)
+            normal = np.zeros((H, W, 3), np.float32) if normal is None else np.ascontiguousarray(normal, np.float32)
+        st = abi.RenderStats()
+        c, r = cam.desc(), rs.desc()
+        self.ctx.check(self.lib.aurora_render(self.h, C.byref(c), C.byref(r), rgb.ctypes.data,
+                                              albedo.ctypes.data if rs.aov else None,
+                                              normal.ctypes.data if rs.aov else None, C.byref(st)))
+        return rgb, albedo, normal, stats_dict(st)
+
+    def render_device(self, cam: Camera, rs: RenderSettings, rgb, albedo=None, normal=None, stats=True):
+        """Device render into torch tensors (H, W, 3) float32 on the context's
+        device; synchronous on return when stats are requested."""
+        st = abi.RenderStats()
+        c, r = cam.desc(), rs.desc()
+        self.ctx.check(self.lib.aurora_render_device(
+            self.h, C.byref(c), C.byref(r), rgb.data_ptr(),
+            albedo.data_ptr() if albedo is not None else None,
+            normal.data_ptr() if normal is not None else None, C.byref(st) if stats else None))
+        return stats_dict(st) if stats else None
+
+
+def resolve_rgb8(rgb_sum, spp):
+    lib = abi.load_library()
+    a = np.ascontiguousarray(rgb_sum, np.float32)
+    H, W = a.shape[:2]
+    out = np.zeros((H, W, 3), np.uint8)
+    rc = lib.aurora_resolve_rgb8(a.ctypes.data, W, H, spp, out.ctypes.data)
+    if rc != abi.AURORA_OK:
+        raise ValueError("aurora_resolve_rgb8: invalid arguments")
+    return out
+
+
+def write_ppm(path, rgb8):
+    """P6 writer with the byte layout of are::Texture::save_texture
+    (src/texture.cpp:370-391): header 'P6\\n<w> <h>\\n255\\n', rows top-down."""
+    H, W = rgb8.shape[:2]
+    with open(path, "wb") as f:
+        f.write(b"P6\n%d %d\n255\n" % (W, H))
+        f.write(np.ascontiguousarray(rgb8, np.uint8).tobytes())

@@ -1,0 +1,287 @@
+"""Workload scenes, flattened to SceneArrays.
+
+Every random draw comes from Philox4x32-10 keyed (seed, STREAM_SCENE), in a
+documented order, so a scene is a pure function of its seed.
+
+  cornell_box()      Cornell box of "Ray Tracing: The Next Week": 555-unit
+                     room, red/green/white lambertian walls, a 130x105 quad
+                     light emitting 15, two white boxes rotated 15 and -18
+                     degrees.  Material constants match the reference's
+                     experiments/rt9.cpp:1446-1451; geometry the 555-unit
+                     room of experiments/rt12.cpp:508-601.
+  random_spheres()   "Ray Tracing in One Weekend" final scene: r=1000 ground,
+                     22x22 jittered grid of r=0.2 spheres (80% lambertian,
+                     15% metal, 5% glass), three r=1 spheres.
+  stress_scene()     traversal stress: N spheres with log-uniform radii and N
+                     random triangles in the unit cube, checker and Perlin
+                     textures.
+  random_rays()      origins uniform in a box, directions uniform on the sphere.
+"""
+import math
+
+import numpy as np
+
+from . import abi
+from .core import Camera, SceneArrays, checker, noise, solid
+
+STREAM_RENDER = 0
+STREAM_SCENE = 1
+
+_M0, _M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+_W0, _W1 = 0x9E3779B9, 0xBB67AE85
+_MASK = np.uint64(0xFFFFFFFF)
+
+
+def philox4x32_10(c0, c1, c2, c3, k0, k1):
+    """Vectorised Philox4x32-10 on uint32 arrays; returns 4 uint32 arrays."""
+    x = [np.asarray(c, dtype=np.uint64) & _MASK for c in (c0, c1, c2, c3)]
+    x = list(np.broadcast_arrays(*x))
+    x = [np.array(v, dtype=np.uint64) for v in x]
+    ka, kb = int(k0) & 0xFFFFFFFF, int(k1) & 0xFFFFFFFF
+    for _ in range(10):
+        p0 = _M0 * x[0]
+        p1 = _M1 * x[2]
+        hi0, lo0 = p0 >> np.uint64(32), p0 & _MASK
+        hi1, lo1 = p1 >> np.uint64(32), p1 & _MASK
+        x = [hi1 ^ x[1] ^ np.uint64(ka), lo1, hi0 ^ x[3] ^ np.uint64(kb), lo0]
+        ka = (ka + _W0) & 0xFFFFFFFF
+        kb = (kb + _W1) & 0xFFFFFFFF
+    return [v.astype(np.uint32) for v in x]
+
+
+def u01(x):
+    """24-bit uniform in [0,1) as float32, exactly as the kernels compute it."""
+    return (np.asarray(x, np.uint32) >> np.uint32(8)).astype(np.float32) * np.float32(1.0 / 16777216.0)
+
+
+class Draws:
+    """Sequential uniforms: draw k is word k%4 of Philox(ctr=(k//4, 0, 0, 0),
+    key=(seed, STREAM_SCENE)), as float64 from the 24-bit float32 value."""
+
+    def __init__(self, seed):
+        self.seed = seed
+        self.k = 0
+        self.buf = []
+
+    def __call__(self, lo=0.0, hi=1.0):
+        if not self.buf:
+            b = self.k // 4
+            r = philox4x32_10(b, 0, 0, 0, self.seed, STREAM_SCENE)
+            self.buf = [float(u01(v)) for v in r]
+        self.k += 1
+        u = self.buf.pop(0)
+        return lo + (hi - lo) * u
+
+    def array(self, n, lo=0.0, hi=1.0):
+        """n draws at once (continues the same sequence)."""
+        if n == 0:
+            return np.zeros(0)
+        assert not self.buf and self.k % 4 == 0, "array() needs a block boundary"
+        nb = (n + 3) // 4
+        b0 = self.k // 4
+        r = philox4x32_10(np.arange(b0, b0 + nb, dtype=np.uint64), 0, 0, 0, self.seed, STREAM_SCENE)
+        u = np.stack([u01(v) for v in r], axis=1).reshape(-1)[:n].astype(np.float64)
+        self.k += 4 * nb
+        return lo + (hi - lo) * u
+
+
+class Builder:
+    """Accumulates primitives; quads carry (Q, u, v) rows."""
+
+    def __init__(self):
+        self.spheres, self.sphere_mat = [], []
+        self.quads, self.quad_mat, self.quad_kind = [], [], []
+        self.materials, self.textures = [], []
+
+    def texture(self, t):
+        self.textures.append(t)
+        return len(self.textures) - 1
+
+    def material(self, mtype, tex=0, fuzz=0.0, ior=1.0):
+        self.materials.append((mtype, tex, fuzz, ior))
+        return len(self.materials) - 1
+
+    def lambertian(self, rgb):
+        return self.material(abi.MAT_LAMBERTIAN, self.texture(solid(*rgb)))
+
+    def metal(self, rgb, fuzz):
+        return self.material(abi.MAT_METAL, self.texture(solid(*rgb)), fuzz=fuzz)
+
+    def dielectric(self, ior):
+        return self.material(abi.MAT_DIELECTRIC, 0, ior=ior)
+
+    def light(self, rgb):
+        return self.material(abi.MAT_DIFFUSE_LIGHT, self.texture(solid(*rgb)))
+
+    def sphere(self, c, r, m):
+        self.spheres.append((c[0], c[1], c[2], r))
+        self.sphere_mat.append(m)
+
+    def quad(self, Q, u, v, m, kind=abi.QUAD_PARALLELOGRAM):
+        self.quads.append(tuple(Q) + tuple(u) + tuple(v))
+        self.quad_mat.append(m)
+        self.quad_kind.append(kind)
+
+    def box(self, a, b, m, rot_y_deg=0.0, offset=(0.0, 0.0, 0.0)):
+        """Six quads of the box [a, b], rotated about +y then translated
+        (rotate_y / translate baked in double precision)."""
+        lo = np.minimum(a, b).astype(np.float64)
+        hi = np.maximum(a, b).astype(np.float64)
+        dx = np.array([hi[0] - lo[0], 0, 0])
+        dy = np.array([0, hi[1] - lo[1], 0])
+        dz = np.array([0, 0, hi[2] - lo[2]])
+        sides = [
+            ((lo[0], lo[1], hi[2]), dx, dy),
+            ((hi[0], lo[1], hi[2]), -dz, dy),
+            ((hi[0], lo[1], lo[2]), -dx, dy),
+            ((lo[0], lo[1], lo[2]), dz, dy),
+            ((lo[0], hi[1], hi[2]), dx, -dz),
+            ((lo[0], lo[1], lo[2]), dx, dz),
+        ]
+        th = math.radians(rot_y_deg)
+        c, s = math.cos(th), math.sin(th)
+
+        def rot(p):
+            return np.array([c * p[0] + s * p[2], p[1], -s * p[0] + c * p[2]])
+
+        off = np.asarray(offset, np.float64)
+        for Q, u, v in sides:
+            self.quad(rot(np.asarray(Q)) + off, rot(u), rot(v), m)
+
+    def arrays(self):
+        return SceneArrays(
+            spheres=np.asarray(self.spheres, np.float32).reshape(-1, 4),
+            sphere_mat=np.asarray(self.sphere_mat, np.uint32),
+            quads=np.asarray(self.quads, np.float32).reshape(-1, 9),
+            quad_mat=np.asarray(self.quad_mat, np.uint32),
+            quad_kind=np.asarray(self.quad_kind, np.uint32),
+            materials=list(self.materials),
+            textures=list(self.textures),
+        )
+
+
+def cornell_box(box_rot=(15.0, -18.0), box_size=((165, 330, 165), (165, 165, 165)),
+                albedo_white=(0.73, 0.73, 0.73), albedo_red=(0.65, 0.05, 0.05), albedo_green=(0.12, 0.45, 0.15),
+                light_size=(130.0, 105.0), emit=15.0):
+    b = Builder()
+    red = b.lambertian(albedo_red)
+    white = b.lambertian(albedo_white)
+    green = b.lambertian(albedo_green)
+    light = b.light((emit, emit, emit))
+    lw, ld = light_size
+    b.quad((555, 0, 0), (0, 555, 0), (0, 0, 555), green)
+    b.quad((0, 0, 0), (0, 555, 0), (0, 0, 555), red)
+    b.quad((278 + lw / 2, 554, 279.5 + ld / 2), (-lw, 0, 0), (0, 0, -ld), light)
+    b.quad((0, 0, 0), (555, 0, 0), (0, 0, 555), white)
+    b.quad((555, 555, 555), (-555, 0, 0), (0, 0, -555), white)
+    b.quad((0, 0, 555), (555, 0, 0), (0, 555, 0), white)
+    b.box((0, 0, 0), box_size[0], white, box_rot[0], (265, 0, 295))
+    b.box((0, 0, 0), box_size[1], white, box_rot[1], (130, 0, 65))
+    return b.arrays()
+
+
+def cornell_camera(width=600, height=None):
+    return Camera(width, height or width, 40.0, (278.0, 278.0, -800.0), (278.0, 278.0, 0.0), (0.0, 1.0, 0.0),
+                  0.0, 10.0)
+
+
+def random_spheres(seed=0):
+    rnd = Draws(seed)
+    b = Builder()
+    b.sphere((0, -1000, 0), 1000, b.lambertian((0.5, 0.5, 0.5)))
+    for a in range(-11, 11):
+        for bb in range(-11, 11):
+            choose = rnd()
+            cx, cz = a + 0.9 * rnd(), bb + 0.9 * rnd()
+            if math.sqrt((cx - 4) ** 2 + 0 + cz ** 2) <= 0.9:
+                continue
+            if choose < 0.8:
+                c1 = (rnd(), rnd(), rnd())
+                c2 = (rnd(), rnd(), rnd())
+                m = b.lambertian(tuple(x * y for x, y in zip(c1, c2)))
+            elif choose < 0.95:
+                alb = (rnd(0.5, 1), rnd(0.5, 1), rnd(0.5, 1))
+                m = b.metal(alb, rnd(0, 0.5))
+            else:
+                m = b.dielectric(1.5)
+            b.sphere((cx, 0.2, cz), 0.2, m)
+    b.sphere((0, 1, 0), 1.0, b.dielectric(1.5))
+    b.sphere((-4, 1, 0), 1.0, b.lambertian((0.4, 0.2, 0.1)))
+    b.sphere((4, 1, 0), 1.0, b.metal((0.7, 0.6, 0.5), 0.0))
+    return b.arrays()
+
+
+def random_spheres_camera(width=1200, height=675):
+    return Camera(width, height, 20.0, (13.0, 2.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 0.6, 10.0)
+
+
+def stress_scene(n_spheres=2_000_000, n_tris=2_000_000, seed=0):
+    """Traversal stress test.  Draw order: sphere centres (3n), log10 radii
+    (n), triangle origins (3m), u directions (3m), u lengths (m), v
+    directions (3m), v lengths (m), material choice (n+m); directions are
+    normalised Gaussian-free cube samples (rejected if |d|<1e-3)."""
+    rnd = Draws(seed)
+    cen = rnd.array(3 * n_spheres).reshape(-1, 3)
+    rad = 10.0 ** rnd.array(n_spheres, -3.0, -1.0)
+    Q = rnd.array(3 * n_tris).reshape(-1, 3)
+
+    def dirs(m):
+        d = rnd.array(3 * m, -1.0, 1.0).reshape(-1, 3)
+        l = np.linalg.norm(d, axis=1, keepdims=True)
+        d = np.where(l < 1e-3, np.array([[1.0, 0.0, 0.0]]), d)
+        return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+    ud = dirs(n_tris)
+    ul = rnd.array(n_tris, 0.001, 0.05)
+    vd = dirs(n_tris)
+    vl = rnd.array(n_tris, 0.001, 0.05)
+    u = ud * ul[:, None]
+    v = vd * vl[:, None]
+    # reject near-collinear edges: replace v by u x (previous v) direction
+    cr = np.cross(u, v)
+    bad = np.linalg.norm(cr, axis=1) < 1e-7
+    if bad.any():
+        alt = np.cross(u[bad], np.array([0.0, 0.0, 1.0]))
+        alt[np.linalg.norm(alt, axis=1) < 1e-9] = np.array([0.0, 1.0, 0.0]) * 1e-2
+        v[bad] = alt / np.linalg.norm(alt, axis=1, keepdims=True) * vl[bad, None]
+    choice = rnd.array(n_spheres + n_tris)
+    b = Builder()
+    t_even = b.texture(solid(0.2, 0.3, 0.1))
+    t_odd = b.texture(solid(0.9, 0.9, 0.9))
+    t_chk = b.texture(checker(0.02, t_even, t_odd))
+    t_noise = b.texture(noise(40.0, seed=0))
+    m_chk = b.material(abi.MAT_LAMBERTIAN, t_chk)
+    m_noise = b.material(abi.MAT_LAMBERTIAN, t_noise)
+    mats = np.where(choice < 0.5, m_chk, m_noise).astype(np.uint32)
+    arr = b.arrays()
+    arr.spheres = np.concatenate([cen, rad[:, None]], axis=1).astype(np.float32)
+    arr.sphere_mat = mats[:n_spheres]
+    arr.quads = np.concatenate([Q, u, v], axis=1).astype(np.float32)
+    arr.quad_mat = mats[n_spheres:]
+    arr.quad_kind = np.full(n_tris, abi.QUAD_TRIANGLE, np.uint32)
+    return arr
+
+
+def stress_camera(width=1920, height=1080):
+    return Camera(width, height, 45.0, (0.5, 0.5, -1.2), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 0.0, 1.7)
+
+
+def random_rays(n, lo, hi, seed=0):
+    """n rays, origins uniform in [lo, hi]^3 and directions uniform on the
+    unit sphere (Philox(i, 1, 0, 0) / (i, 2, 0, 0) with key (seed,
+    STREAM_SCENE); normal deviates via Box-Muller in float64)."""
+    i = np.arange(n, dtype=np.uint64)
+    a = philox4x32_10(i, 1, 0, 0, seed, STREAM_SCENE)
+    c = philox4x32_10(i, 2, 0, 0, seed, STREAM_SCENE)
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    o = lo + (hi - lo) * np.stack([u01(a[0]), u01(a[1]), u01(a[2])], axis=1).astype(np.float64)
+    u1 = 1.0 - u01(c[0]).astype(np.float64)
+    u2 = u01(c[1]).astype(np.float64)
+    u3 = 1.0 - u01(c[2]).astype(np.float64)
+    u4 = u01(c[3]).astype(np.float64)
+    r1 = np.sqrt(-2 * np.log(u1))
+    r2 = np.sqrt(-2 * np.log(u3))
+    g = np.stack([r1 * np.cos(2 * np.pi * u2), r1 * np.sin(2 * np.pi * u2), r2 * np.cos(2 * np.pi * u4)], axis=1)
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    return o.astype(np.float32), g.astype(np.float32)

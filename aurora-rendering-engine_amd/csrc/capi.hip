@@ -1,0 +1,718 @@
+// capi.hip — host side of libaurora's C ABI (include/aurora.h).
+//
+// Owns contexts (device + stream), uploads and validates flattened scenes,
+// builds the LBVH, and drives the wavefront passes (raygen, then per bounce
+// extend + shade, then accumulate).  Never throws across the ABI: every
+// failure becomes a status code plus a message in aurora_last_error.
+//
+// Reference counterparts: scene containers are::ObjectSet / are::Triangle
+// (include/object/object_set.h:10-12, include/object/triangle.h:19,38-39);
+// validation rules of the Triangle constructor (src/object/triangle.cpp:
+// 13-36); the render loop of experiments/rt.cpp:337-374; the P6 byte layout
+// of are::Texture::save_texture (src/texture.cpp:378-391).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace aur;
+
+struct aurora_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cus = 0;
+    int ext_grid = 0;    // resident blocks of the extend kernel
+    int shade_grid = 0;
+};
+
+struct aurora_scene {
+    aurora_ctx* ctx = nullptr;
+    DevScene view{};
+    uint32_t n_prims = 0;
+    DevBuf spheres, sphere_mat, quads, mats, texs, texels, perlin_vec, perlin_perm;
+    DevBuf nodes, morton, order;
+    BvhScratch scr;
+    bool built = false;
+    aurora_bvh_stats stats{};
+    Workspace ws;
+    DevBuf tr_o, tr_d, tr_hit, tr_spill, tr_pool, totals;
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int fail(aurora_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+int fail(aurora_ctx* c, const Status& s) { return fail(c, s.code, s.msg); }
+
+#define CK(ctx, expr)                         \
+    do {                                      \
+        Status st_ = (expr);                  \
+        if (st_.bad()) return fail(ctx, st_); \
+    } while (0)
+
+Status hip_status(hipError_t e, const char* what) {
+    if (e == hipSuccess) return Status::ok();
+    return Status{e == hipErrorOutOfMemory ? AURORA_ERR_NOMEM : AURORA_ERR_RUNTIME,
+                  std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+Status upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
+    AUR_TRY(b.alloc(bytes ? bytes : 16));
+    if (bytes) AUR_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+    return Status::ok();
+}
+
+bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+bool tiny3(double x, double y, double z) { return std::fabs(x) < 1e-8 && std::fabs(y) < 1e-8 && std::fabs(z) < 1e-8; }
+
+std::string check_scene(const aurora_scene_desc& d) {
+    char b[200];
+    if (d.n_spheres && (!d.spheres || !d.sphere_material)) return "sphere arrays are NULL";
+    if (d.n_quads && (!d.quads || !d.quad_material)) return "quad arrays are NULL";
+    if (d.n_materials && !d.materials) return "material table is NULL";
+    if (d.n_textures && !d.textures) return "texture table is NULL";
+    if (d.n_texels && !d.texels) return "texel array is NULL";
+    if ((uint64_t)d.n_spheres + d.n_quads > 0x7fffffffull) return "too many primitives";
+    for (uint32_t i = 0; i < d.n_spheres; ++i) {
+        const float* s = d.spheres + 4 * (size_t)i;
+        if (!finite3(s) || !std::isfinite(s[3]) || !(s[3] > 0.0f))
+            return (snprintf(b, sizeof b, "sphere %u: non-finite centre or radius <= 0", i), b);
+        if (d.sphere_material[i] >= d.n_materials)
+            return (snprintf(b, sizeof b, "sphere %u: material index out of range", i), b);
+    }
+    for (uint32_t i = 0; i < d.n_quads; ++i) {
+        const float* q = d.quads + 9 * (size_t)i;
+        if (!finite3(q) || !finite3(q + 3) || !finite3(q + 6))
+            return (snprintf(b, sizeof b, "quad %u: non-finite vertex or edge", i), b);
+        const double ux = q[3], uy = q[4], uz = q[5], vx = q[6], vy = q[7], vz = q[8];
+        if (tiny3(ux, uy, uz)) return (snprintf(b, sizeof b, "quad %u: edge vector u cannot be zero vector", i), b);
+        if (tiny3(vx, vy, vz)) return (snprintf(b, sizeof b, "quad %u: edge vector v cannot be zero vector", i), b);
+        if (tiny3(uy * vz - uz * vy, uz * vx - ux * vz, ux * vy - uy * vx))
+            return (snprintf(b, sizeof b, "quad %u: edge vectors u and v cannot be collinear", i), b);
+        if (d.quad_material[i] >= d.n_materials)
+            return (snprintf(b, sizeof b, "quad %u: material index out of range", i), b);
+        if (d.quad_kind && d.quad_kind[i] > AURORA_QUAD_TRIANGLE)
+            return (snprintf(b, sizeof b, "quad %u: unknown kind", i), b);
+    }
+    for (uint32_t i = 0; i < d.n_materials; ++i) {
+        const aurora_material& m = d.materials[i];
+        if (m.type > AURORA_MAT_DIFFUSE_LIGHT) return (snprintf(b, sizeof b, "material %u: unknown type", i), b);
+        if (m.type != AURORA_MAT_DIELECTRIC && m.texture >= d.n_textures)
+            return (snprintf(b, sizeof b, "material %u: texture index out of range", i), b);
+        if (m.type == AURORA_MAT_DIELECTRIC && !(m.ior > 0.0f && std::isfinite(m.ior)))
+            return (snprintf(b, sizeof b, "material %u: ior must be positive", i), b);
+        if (m.type == AURORA_MAT_METAL && !(m.fuzz >= 0.0f))
+            return (snprintf(b, sizeof b, "material %u: fuzz must be >= 0", i), b);
+    }
+    for (uint32_t i = 0; i < d.n_textures; ++i) {
+        const aurora_texture& t = d.textures[i];
+        if (t.type > AURORA_TEX_IMAGE) return (snprintf(b, sizeof b, "texture %u: unknown type", i), b);
+        if (t.type == AURORA_TEX_CHECKER) {
+            if (t.even >= d.n_textures || t.odd >= d.n_textures)
+                return (snprintf(b, sizeof b, "texture %u: checker child out of range", i), b);
+            if (d.textures[t.even].type == AURORA_TEX_CHECKER || d.textures[t.odd].type == AURORA_TEX_CHECKER)
+                return (snprintf(b, sizeof b, "texture %u: checker children cannot be checkers", i), b);
+            if (!(t.scale > 0.0f)) return (snprintf(b, sizeof b, "texture %u: checker scale must be positive", i), b);
+        }
+        if (t.type == AURORA_TEX_IMAGE) {
+            if (t.width == 0 || t.height == 0)
+                return (snprintf(b, sizeof b, "texture %u: image width and height must be positive", i), b);
+            if (t.texel_offset + (uint64_t)t.width * t.height > d.n_texels)
+                return (snprintf(b, sizeof b, "texture %u: image texels out of range", i), b);
+        }
+    }
+    return "";
+}
+
+// Perlin tables from Philox(seed, STREAM_PERLIN): 256 unit gradients from one
+// draw each, then three Fisher–Yates shuffles (index j = floor(x*(i+1)/2^32)).
+void perlin_fill(uint32_t seed, float* vec, int* perm) {
+    for (uint32_t i = 0; i < 256; ++i) {
+        const u4 r = philox(u4{i, 0u, 0u, 0u}, seed, STREAM_PERLIN);
+        float x = 2.0f * u01(r.x) - 1.0f, y = 2.0f * u01(r.y) - 1.0f, z = 2.0f * u01(r.z) - 1.0f;
+        if (x * x + y * y + z * z == 0.0f) { x = 1.0f; y = 0.0f; z = 0.0f; }
+        const float l = sqrtf(x * x + y * y + z * z);
+        vec[3 * i] = x / l;
+        vec[3 * i + 1] = y / l;
+        vec[3 * i + 2] = z / l;
+    }
+    for (uint32_t a = 0; a < 3; ++a) {
+        int* p = perm + 256 * a;
+        for (int i = 0; i < 256; ++i) p[i] = i;
+        for (uint32_t i = 255; i >= 1; --i) {
+            const u4 r = philox(u4{i, a + 1u, 0u, 0u}, seed, STREAM_PERLIN);
+            const uint32_t j = (uint32_t)(((uint64_t)r.x * (i + 1u)) >> 32);
+            const int t = p[i];
+            p[i] = p[j];
+            p[j] = t;
+        }
+    }
+}
+
+struct D3 {
+    double x, y, z;
+};
+D3 operator-(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+D3 operator+(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+D3 operator*(double s, D3 a) { return D3{s * a.x, s * a.y, s * a.z}; }
+D3 operator/(D3 a, double s) { return D3{a.x / s, a.y / s, a.z / s}; }
+double len(D3 a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+D3 crs(D3 a, D3 b) { return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+v3 tof(D3 a) { return V((float)a.x, (float)a.y, (float)a.z); }
+
+// Camera frame in double, then rounded once to float ("Ray Tracing in One
+// Weekend" camera: viewport at focus_dist, pixel centres at +0.5 as in
+// experiments/rt.cpp:364-366).
+std::string camera_frame(const aurora_camera_desc& c, CamF& out) {
+    if (c.image_width == 0 || c.image_height == 0) return "image width and height must be positive";
+    if ((uint64_t)c.image_width * c.image_height > (1ull << 31)) return "image too large";
+    if (!(c.vfov > 0.0f && c.vfov < 180.0f)) return "vfov must be in (0, 180) degrees";
+    if (!(c.focus_dist > 0.0f)) return "focus_dist must be positive";
+    const double PI = 3.1415926535897932385;
+    const D3 lf{c.lookfrom[0], c.lookfrom[1], c.lookfrom[2]}, la{c.lookat[0], c.lookat[1], c.lookat[2]},
+        up{c.vup[0], c.vup[1], c.vup[2]};
+    if (len(lf - la) == 0.0) return "lookfrom and lookat coincide";
+    const double theta = (double)c.vfov * PI / 180.0;
+    const double h = std::tan(theta / 2.0);
+    const double fd = c.focus_dist;
+    const double vh = 2.0 * h * fd;
+    const double vw = vh * ((double)c.image_width / (double)c.image_height);
+    const D3 w = (lf - la) / len(lf - la);
+    const D3 cu = crs(up, w);
+    if (len(cu) == 0.0) return "vup is parallel to the view direction";
+    const D3 u = cu / len(cu);
+    const D3 v = crs(w, u);
+    const D3 vu = vw * u, vv = vh * D3{-v.x, -v.y, -v.z};
+    const D3 du = vu / (double)c.image_width, dv = vv / (double)c.image_height;
+    const D3 ul = ((lf - fd * w) - vu / 2.0) - vv / 2.0;
+    const D3 p00 = ul + 0.5 * (du + dv);
+    const double rad = fd * std::tan(((double)c.defocus_angle / 2.0) * PI / 180.0);
+    out.center = tof(lf);
+    out.p00 = tof(p00);
+    out.du = tof(du);
+    out.dv = tof(dv);
+    out.disk_u = tof(rad * u);
+    out.disk_v = tof(rad * v);
+    out.defocus = c.defocus_angle > 0.0f ? 1u : 0u;
+    out.W = c.image_width;
+    out.H = c.image_height;
+    return "";
+}
+
+__global__ void k_pack3(const float* src, float4* dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = make_float4(src[3 * (size_t)i], src[3 * (size_t)i + 1], src[3 * (size_t)i + 2], 0.0f);
+}
+__global__ void k_unpack_hits(const float2* h, int32_t* prim, float* t, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const float2 v = h[i];
+        prim[i] = __float_as_int(v.y);
+        t[i] = v.x;
+    }
+}
+
+int grid_for_extend(aurora_ctx* c, uint64_t n) {
+    const uint64_t need = (n + extend_block() - 1) / extend_block();
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(need, (uint64_t)c->ext_grid));
+}
+
+Status ensure_spill(DevBuf& spill, int grid) {
+    return spill.alloc((size_t)grid * extend_block() * extend_spill_levels() * 4);
+}
+
+Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n, float tmin, float tmax,
+                    int32_t* prim, float* t, uint32_t* counts, bool brute) {
+    aurora_ctx* c = s->ctx;
+    if (!brute && !s->built) return Status{AURORA_ERR_STATE, "BVH not built: call aurora_bvh_build first"};
+    if (n == 0) return Status::ok();
+    AUR_TRY(s->tr_o.alloc((size_t)n * 16));
+    AUR_TRY(s->tr_d.alloc((size_t)n * 16));
+    AUR_TRY(s->tr_hit.alloc((size_t)n * 8));
+    const uint32_t blocks = (n + 255) / 256;
+    k_pack3<<<blocks, 256, 0, c->stream>>>(o, s->tr_o.as<float4>(), n);
+    k_pack3<<<blocks, 256, 0, c->stream>>>(d, s->tr_d.as<float4>(), n);
+    AUR_HIP(hipGetLastError());
+    if (brute) {
+        AUR_TRY(launch_brute(s->view, s->tr_o.as<float4>(), s->tr_d.as<float4>(), n, tmin, tmax,
+                             s->tr_hit.as<float2>(), c->stream));
+    } else {
+        const int grid = grid_for_extend(c, n);
+        AUR_TRY(ensure_spill(s->tr_spill, grid));
+        AUR_TRY(s->tr_pool.alloc(16));
+        AUR_HIP(hipMemsetAsync(s->tr_pool.p, 0, 4, c->stream));
+        ExtendArgs a{};
+        a.ray_o = s->tr_o.as<float4>();
+        a.ray_d = s->tr_d.as<float4>();
+        a.count = nullptr;
+        a.n = n;
+        a.pool = s->tr_pool.as<uint32_t>();
+        a.tmin = tmin;
+        a.tmax = tmax;
+        a.hit = s->tr_hit.as<float2>();
+        a.counts = counts;
+        a.spill = s->tr_spill.as<uint32_t>();
+        a.spill_stride = (uint32_t)(grid * extend_block());
+        AUR_TRY(launch_extend(s->view, a, grid, c->stream));
+    }
+    k_unpack_hits<<<blocks, 256, 0, c->stream>>>(s->tr_hit.as<float2>(), prim, t, n);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status ensure_workspace(aurora_scene* s, uint32_t cap, int ext_grid) {
+    Workspace& w = s->ws;
+    if (cap > w.capacity) {
+        for (int k = 0; k < 2; ++k) {
+            AUR_TRY(w.q_o[k].alloc((size_t)cap * 16));
+            AUR_TRY(w.q_d[k].alloc((size_t)cap * 16));
+            AUR_TRY(w.q_b[k].alloc((size_t)cap * 16));
+        }
+        AUR_TRY(w.hit.alloc((size_t)cap * 8));
+        AUR_TRY(w.L.alloc((size_t)cap * 16));
+        AUR_TRY(w.aov_a.alloc((size_t)cap * 16));
+        AUR_TRY(w.aov_n.alloc((size_t)cap * 16));
+        w.capacity = cap;
+    }
+    AUR_TRY(w.counters.alloc(64));
+    AUR_TRY(ensure_spill(w.spill, ext_grid));
+    if (!w.host_counts) AUR_HIP(hipHostMalloc((void**)&w.host_counts, 256 * sizeof(uint32_t), hipHostMallocDefault));
+    return Status::ok();
+}
+
+constexpr uint32_t kPassPaths = 1u << 23;  // wavefront width per pass
+
+Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const aurora_render_desc* rd, float* rgb,
+                     float* alb, float* nrm, aurora_render_stats* stats, bool count_fetches) {
+    aurora_ctx* c = s->ctx;
+    if (!cam || !rd) return Status{AURORA_ERR_INVALID, "null camera or render descriptor"};
+    if (!rgb) return Status{AURORA_ERR_INVALID, "rgb_sum is NULL"};
+    RenderArgs r{};
+    const std::string e = camera_frame(*cam, r.cam);
+    if (!e.empty()) return Status{AURORA_ERR_INVALID, e};
+    if (rd->max_depth == 0) return Status{AURORA_ERR_INVALID, "max_depth must be >= 1"};
+    if (rd->max_depth > 250) return Status{AURORA_ERR_INVALID, "max_depth must be <= 250"};
+    if (!finite3(rd->background)) return Status{AURORA_ERR_INVALID, "background must be finite"};
+    if ((rd->flags & AURORA_RENDER_AOV) && (!alb || !nrm))
+        return Status{AURORA_ERR_INVALID, "AOV flag set but albedo/normal buffers are NULL"};
+    if (!s->built) return Status{AURORA_ERR_STATE, "BVH not built: call aurora_bvh_build first"};
+    if ((uint64_t)rd->sample_offset + rd->spp > 0xffffffffull) return Status{AURORA_ERR_INVALID, "sample index overflow"};
+    aurora_render_stats st{};
+    const uint32_t WH = r.cam.W * r.cam.H;
+    uint32_t S = rd->spp_per_pass;
+    if (S == 0) S = std::max<uint32_t>(1, kPassPaths / WH);
+    S = std::min<uint32_t>(S, std::max<uint32_t>(1, rd->spp));
+    if ((uint64_t)S * WH > 0xffffffffull) return Status{AURORA_ERR_INVALID, "spp_per_pass * pixels exceeds 2^32"};
+    if (rd->spp == 0) {
+        if (stats) *stats = st;
+        return Status::ok();
+    }
+    const int ext_grid = c->ext_grid;
+    AUR_TRY(ensure_workspace(s, S * WH, ext_grid));
+    Workspace& w = s->ws;
+    r.seed = rd->seed;
+    r.max_depth = rd->max_depth;
+    r.tmin = rd->tmin > 0.0f ? rd->tmin : 0.001f;
+    r.bg = V(rd->background[0], rd->background[1], rd->background[2]);
+    r.aov = (rd->flags & AURORA_RENDER_AOV) != 0;
+    if (count_fetches) {
+        AUR_TRY(s->totals.alloc(64));
+        AUR_HIP(hipMemsetAsync(s->totals.p, 0, 24, c->stream));
+    }
+
+    const bool timed = stats != nullptr;
+    std::vector<hipEvent_t> evs;
+    auto ev = [&](hipEvent_t& e) -> Status {
+        AUR_HIP(hipEventCreate(&e));
+        evs.push_back(e);
+        return Status::ok();
+    };
+    hipEvent_t t_begin = nullptr, t_end = nullptr;
+    if (timed) {
+        AUR_TRY(ev(t_begin));
+        AUR_TRY(ev(t_end));
+        AUR_HIP(hipEventRecord(t_begin, c->stream));
+    }
+    std::vector<hipEvent_t> ext_ev;  // begin/end per extend launch
+    std::vector<hipEvent_t> cnt_ev(rd->max_depth);
+    for (auto& e2 : cnt_ev) AUR_TRY(ev(e2));
+    Status result = Status::ok();
+    for (uint32_t done = 0; done < rd->spp && !result.bad(); done += S) {
+        const uint32_t sp = std::min(S, rd->spp - done);
+        r.spp_pass = sp;
+        r.sample_base = rd->sample_offset + done;
+        const uint32_t n = sp * WH;
+        st.passes++;
+        st.paths += n;
+        st.max_live = std::max(st.max_live, n);
+        AUR_TRY(launch_raygen(s->view, r, w, n, c->stream));
+        uint64_t segs = n;
+        uint32_t launched = 0;
+        for (uint32_t depth = 0; depth < rd->max_depth; ++depth) {
+            const int cur = depth & 1;
+            ExtendArgs a{};
+            a.ray_o = w.q_o[cur].as<float4>();
+            a.ray_d = w.q_d[cur].as<float4>();
+            a.count = w.counters.as<uint32_t>() + cur;
+            a.n = 0;
+            a.pool = w.counters.as<uint32_t>() + 2;
+            a.zero_after = w.counters.as<uint32_t>() + (cur ^ 1);
+            a.tmin = r.tmin;
+            a.tmax = INFINITY;
+            a.hit = w.hit.as<float2>();
+            a.totals = count_fetches ? s->totals.as<unsigned long long>() : nullptr;
+            a.spill = w.spill.as<uint32_t>();
+            a.spill_stride = (uint32_t)(ext_grid * extend_block());
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timed) {
+                AUR_TRY(ev(e0));
+                AUR_TRY(ev(e1));
+                AUR_HIP(hipEventRecord(e0, c->stream));
+            }
+            AUR_TRY(launch_extend(s->view, a, ext_grid, c->stream));
+            if (timed) {
+                AUR_HIP(hipEventRecord(e1, c->stream));
+                ext_ev.push_back(e0);
+                ext_ev.push_back(e1);
+            }
+            AUR_TRY(launch_shade(s->view, r, w, cur, depth, c->shade_grid, c->stream));
+            AUR_HIP(hipMemcpyAsync(w.host_counts + depth, w.counters.as<uint32_t>() + (cur ^ 1), 4,
+                                   hipMemcpyDeviceToHost, c->stream));
+            AUR_HIP(hipEventRecord(cnt_ev[depth], c->stream));
+            launched = depth + 1;
+            // stop once a queue two bounces back came out empty (keeps two
+            // bounces in flight while the host checks)
+            if (depth >= 2) {
+                AUR_HIP(hipEventSynchronize(cnt_ev[depth - 2]));
+                if (w.host_counts[depth - 2] == 0) break;
+            }
+        }
+        AUR_TRY(launch_accumulate(r, w, rgb, alb, nrm, c->stream));
+        AUR_HIP(hipStreamSynchronize(c->stream));
+        for (uint32_t k = 0; k < launched; ++k) segs += w.host_counts[k];
+        st.segments += segs;
+    }
+    if (timed) {
+        AUR_HIP(hipEventRecord(t_end, c->stream));
+        AUR_HIP(hipEventSynchronize(t_end));
+        float ms = 0.0f;
+        AUR_HIP(hipEventElapsedTime(&ms, t_begin, t_end));
+        st.kernel_ms = ms;
+        double ext = 0.0;
+        for (size_t k = 0; k + 1 < ext_ev.size(); k += 2) {
+            float m2 = 0.0f;
+            AUR_HIP(hipEventElapsedTime(&m2, ext_ev[k], ext_ev[k + 1]));
+            ext += m2;
+        }
+        st.extend_ms = ext;
+        st.extend_launches = (uint32_t)(ext_ev.size() / 2);
+    }
+    if (count_fetches) {
+        unsigned long long tot[3];
+        AUR_HIP(hipMemcpyAsync(tot, s->totals.p, 24, hipMemcpyDeviceToHost, c->stream));
+        AUR_HIP(hipStreamSynchronize(c->stream));
+        st.nodes_fetched = tot[0];
+        st.spheres_fetched = tot[1];
+        st.quads_fetched = tot[2];
+    }
+    for (auto& e2 : evs) (void)hipEventDestroy(e2);
+    if (stats) *stats = st;
+    return result;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aurora_abi_version(void) { return AURORA_ABI_VERSION; }
+
+int aurora_create(int device, aurora_ctx** out) {
+    if (!out) return fail(nullptr, AURORA_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+        return fail(nullptr, AURORA_ERR_RUNTIME, std::string("no HIP device: ") + hipGetErrorString(e));
+    if (device < 0 || device >= n) return fail(nullptr, AURORA_ERR_INVALID, "device ordinal out of range");
+    auto* c = new aurora_ctx();
+    c->device = device;
+    Status s = hip_status(hipSetDevice(device), "hipSetDevice");
+    hipDeviceProp_t prop{};
+    if (!s.bad()) s = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    if (!s.bad()) s = hip_status(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (s.bad()) {
+        delete c;
+        return fail(nullptr, s);
+    }
+    c->cus = prop.multiProcessorCount;
+    c->ext_grid = c->cus * 8;
+    c->shade_grid = c->cus * 8;
+    *out = c;
+    return AURORA_OK;
+}
+
+void aurora_destroy(aurora_ctx* c) {
+    if (!c) return;
+    if (c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+}
+
+const char* aurora_last_error(const aurora_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int aurora_synchronize(aurora_ctx* c) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    CK(c, hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    return AURORA_OK;
+}
+
+void* aurora_stream(aurora_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int aurora_set_launch(aurora_ctx* c, int ext_blocks_per_cu, int shade_blocks_per_cu) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (ext_blocks_per_cu < 1 || ext_blocks_per_cu > 64 || shade_blocks_per_cu < 1 || shade_blocks_per_cu > 64)
+        return fail(c, AURORA_ERR_INVALID, "blocks per CU must be in [1, 64]");
+    c->ext_grid = c->cus * ext_blocks_per_cu;
+    c->shade_grid = c->cus * shade_blocks_per_cu;
+    return AURORA_OK;
+}
+
+int aurora_scene_upload(aurora_ctx* c, const aurora_scene_desc* d, aurora_scene** out) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (!d || !out) return fail(c, AURORA_ERR_INVALID, "desc or out is NULL");
+    *out = nullptr;
+    const std::string e = check_scene(*d);
+    if (!e.empty()) return fail(c, AURORA_ERR_INVALID, e);
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    auto* s = new aurora_scene();
+    s->ctx = c;
+    auto cleanup = [&](const Status& st) {
+        aurora_scene_destroy(s);
+        return fail(c, st);
+    };
+    const hipStream_t st = c->stream;
+    // quads: (Q, material) (u, kind) (v, 0)
+    std::vector<float4> q((size_t)d->n_quads * 3);
+    for (uint32_t i = 0; i < d->n_quads; ++i) {
+        const float* p = d->quads + 9 * (size_t)i;
+        uint32_t kind = d->quad_kind ? d->quad_kind[i] : AURORA_QUAD_PARALLELOGRAM;
+        uint32_t mat = d->quad_material[i];
+        float fm, fk;
+        memcpy(&fm, &mat, 4);
+        memcpy(&fk, &kind, 4);
+        q[3 * (size_t)i] = make_float4(p[0], p[1], p[2], fm);
+        q[3 * (size_t)i + 1] = make_float4(p[3], p[4], p[5], fk);
+        q[3 * (size_t)i + 2] = make_float4(p[6], p[7], p[8], 0.0f);
+    }
+    std::vector<aurora_material> mats(d->materials, d->materials + d->n_materials);
+    for (auto& m : mats)
+        if (m.type == AURORA_MAT_METAL && m.fuzz > 1.0f) m.fuzz = 1.0f;
+    std::vector<aurora_texture> texs(d->textures, d->textures + d->n_textures);
+    uint32_t slots = 0;
+    for (auto& t : texs)
+        if (t.type == AURORA_TEX_NOISE) t.even = slots++;
+    std::vector<float> pv((size_t)std::max(1u, slots) * 768);
+    std::vector<int> pp((size_t)std::max(1u, slots) * 768);
+    for (auto& t : texs)
+        if (t.type == AURORA_TEX_NOISE) perlin_fill(t.seed, &pv[(size_t)t.even * 768], &pp[(size_t)t.even * 768]);
+    Status r = upload(s->spheres, d->spheres, (size_t)d->n_spheres * 16, st);
+    if (!r.bad()) r = upload(s->sphere_mat, d->sphere_material, (size_t)d->n_spheres * 4, st);
+    if (!r.bad()) r = upload(s->quads, q.data(), q.size() * 16, st);
+    if (!r.bad()) r = upload(s->mats, mats.data(), mats.size() * sizeof(aurora_material), st);
+    if (!r.bad()) r = upload(s->texs, texs.data(), texs.size() * sizeof(aurora_texture), st);
+    if (!r.bad()) r = upload(s->texels, d->texels, (size_t)d->n_texels * 12, st);
+    if (!r.bad()) r = upload(s->perlin_vec, pv.data(), pv.size() * 4, st);
+    if (!r.bad()) r = upload(s->perlin_perm, pp.data(), pp.size() * 4, st);
+    if (!r.bad()) r = hip_status(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (r.bad()) return cleanup(r);
+    DevScene& v = s->view;
+    v.spheres = s->spheres.as<float4>();
+    v.sphere_mat = s->sphere_mat.as<uint32_t>();
+    v.quads = s->quads.as<float4>();
+    v.mats = s->mats.as<aurora_material>();
+    v.texs = s->texs.as<aurora_texture>();
+    v.texels = s->texels.as<float>();
+    v.perlin_vec = s->perlin_vec.as<float>();
+    v.perlin_perm = s->perlin_perm.as<int>();
+    v.nodes = nullptr;
+    v.n_spheres = d->n_spheres;
+    v.n_quads = d->n_quads;
+    v.n_nodes = 0;
+    s->n_prims = d->n_spheres + d->n_quads;
+    *out = s;
+    return AURORA_OK;
+}
+
+void aurora_scene_destroy(aurora_scene* s) {
+    if (!s) return;
+    if (s->ctx && s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
+    DevBuf* bufs[] = {&s->spheres, &s->sphere_mat, &s->quads, &s->mats, &s->texs, &s->texels, &s->perlin_vec,
+                      &s->perlin_perm, &s->nodes, &s->morton, &s->order, &s->scr.box_lo, &s->scr.box_hi,
+                      &s->scr.bounds, &s->scr.keys[0], &s->scr.keys[1], &s->scr.vals[0], &s->scr.vals[1],
+                      &s->scr.hist, &s->scr.gamma, &s->scr.flags, &s->scr.int_parent, &s->scr.leaf_parent,
+                      &s->scr.int_pos, &s->scr.leaf_pos, &s->scr.arrive, &s->scr.depth, &s->ws.q_o[0],
+                      &s->ws.q_o[1], &s->ws.q_d[0], &s->ws.q_d[1], &s->ws.q_b[0], &s->ws.q_b[1], &s->ws.hit,
+                      &s->ws.L, &s->ws.aov_a, &s->ws.aov_n, &s->ws.counters, &s->ws.spill, &s->tr_o, &s->tr_d,
+                      &s->tr_hit, &s->tr_spill, &s->tr_pool, &s->totals};
+    for (DevBuf* b : bufs) b->release();
+    if (s->ws.host_counts) (void)hipHostFree(s->ws.host_counts);
+    delete s;
+}
+
+int aurora_bvh_build(aurora_scene* s, aurora_bvh_stats* stats) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    aurora_ctx* c = s->ctx;
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    aurora_bvh_stats st{};
+    CK(c, lbvh_build(s->view, s->n_prims, s->nodes, s->morton, s->order, s->scr, c->stream, &st));
+    s->view.nodes = s->nodes.as<float4>();
+    s->view.n_nodes = st.n_nodes;
+    s->stats = st;
+    s->built = true;
+    if (stats) *stats = st;
+    return AURORA_OK;
+}
+
+int aurora_bvh_download(aurora_scene* s, aurora_bvh_node* nodes, uint32_t capacity) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    aurora_ctx* c = s->ctx;
+    if (!s->built) return fail(c, AURORA_ERR_STATE, "BVH not built");
+    if (capacity < s->stats.n_nodes) return fail(c, AURORA_ERR_INVALID, "capacity smaller than the node count");
+    if (s->stats.n_nodes == 0) return AURORA_OK;
+    if (!nodes) return fail(c, AURORA_ERR_INVALID, "nodes is NULL");
+    CK(c, hip_status(hipMemcpyAsync(nodes, s->nodes.p, (size_t)s->stats.n_nodes * 32, hipMemcpyDeviceToHost,
+                                    c->stream),
+                     "hipMemcpyAsync"));
+    CK(c, hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    return AURORA_OK;
+}
+
+int aurora_bvh_download_order(aurora_scene* s, uint32_t* morton, uint32_t* order, uint32_t capacity) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    aurora_ctx* c = s->ctx;
+    if (!s->built) return fail(c, AURORA_ERR_STATE, "BVH not built");
+    if (capacity < s->n_prims) return fail(c, AURORA_ERR_INVALID, "capacity smaller than the primitive count");
+    if (s->n_prims == 0) return AURORA_OK;
+    if (!morton || !order) return fail(c, AURORA_ERR_INVALID, "output is NULL");
+    CK(c, hip_status(hipMemcpyAsync(morton, s->morton.p, (size_t)s->n_prims * 4, hipMemcpyDeviceToHost, c->stream),
+                     "hipMemcpyAsync"));
+    CK(c, hip_status(hipMemcpyAsync(order, s->order.p, (size_t)s->n_prims * 4, hipMemcpyDeviceToHost, c->stream),
+                     "hipMemcpyAsync"));
+    CK(c, hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    return AURORA_OK;
+}
+
+int aurora_trace_closest_device(aurora_scene* s, const float* o, const float* d, uint32_t n, float tmin, float tmax,
+                                int32_t* prim, float* t, uint32_t* counts) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    if (n && (!o || !d || !prim || !t)) return fail(s->ctx, AURORA_ERR_INVALID, "NULL ray or output buffer");
+    CK(s->ctx, hip_status(hipSetDevice(s->ctx->device), "hipSetDevice"));
+    CK(s->ctx, trace_device(s, o, d, n, tmin, tmax, prim, t, counts, false));
+    return AURORA_OK;
+}
+
+int aurora_trace_brute_device(aurora_scene* s, const float* o, const float* d, uint32_t n, float tmin, float tmax,
+                              int32_t* prim, float* t) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    if (n && (!o || !d || !prim || !t)) return fail(s->ctx, AURORA_ERR_INVALID, "NULL ray or output buffer");
+    CK(s->ctx, hip_status(hipSetDevice(s->ctx->device), "hipSetDevice"));
+    CK(s->ctx, trace_device(s, o, d, n, tmin, tmax, prim, t, nullptr, true));
+    return AURORA_OK;
+}
+
+int aurora_trace_closest(aurora_scene* s, const float* o, const float* d, uint32_t n, float tmin, float tmax,
+                         int32_t* prim, float* t) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    aurora_ctx* c = s->ctx;
+    if (n && (!o || !d || !prim || !t)) return fail(c, AURORA_ERR_INVALID, "NULL ray or output buffer");
+    if (!s->built) return fail(c, AURORA_ERR_STATE, "BVH not built: call aurora_bvh_build first");
+    if (n == 0) return AURORA_OK;
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    DevBuf bo, bd, bp, bt;
+    Status r = bo.alloc((size_t)n * 12);
+    if (!r.bad()) r = bd.alloc((size_t)n * 12);
+    if (!r.bad()) r = bp.alloc((size_t)n * 4);
+    if (!r.bad()) r = bt.alloc((size_t)n * 4);
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(bo.p, o, (size_t)n * 12, hipMemcpyHostToDevice, c->stream), "copy");
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(bd.p, d, (size_t)n * 12, hipMemcpyHostToDevice, c->stream), "copy");
+    if (!r.bad()) r = trace_device(s, bo.as<float>(), bd.as<float>(), n, tmin, tmax, bp.as<int32_t>(), bt.as<float>(),
+                                   nullptr, false);
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(prim, bp.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(t, bt.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r.bad()) r = hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    (void)hipStreamSynchronize(c->stream);
+    bo.release();
+    bd.release();
+    bp.release();
+    bt.release();
+    if (r.bad()) return fail(c, r);
+    return AURORA_OK;
+}
+
+int aurora_render_device(aurora_scene* s, const aurora_camera_desc* cam, const aurora_render_desc* rd,
+                         float* rgb, float* alb, float* nrm, aurora_render_stats* stats) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    CK(s->ctx, hip_status(hipSetDevice(s->ctx->device), "hipSetDevice"));
+    const bool count = rd && (rd->flags & AURORA_RENDER_COUNT_FETCHES);
+    CK(s->ctx, render_device(s, cam, rd, rgb, alb, nrm, stats, count));
+    return AURORA_OK;
+}
+
+int aurora_render(aurora_scene* s, const aurora_camera_desc* cam, const aurora_render_desc* rd, float* rgb,
+                  float* alb, float* nrm, aurora_render_stats* stats) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    aurora_ctx* c = s->ctx;
+    if (!cam || !rd || !rgb) return fail(c, AURORA_ERR_INVALID, "null camera, descriptor or rgb_sum");
+    const bool aov = (rd->flags & AURORA_RENDER_AOV) != 0;
+    if (aov && (!alb || !nrm)) return fail(c, AURORA_ERR_INVALID, "AOV flag set but albedo/normal buffers are NULL");
+    if (cam->image_width == 0 || cam->image_height == 0)
+        return fail(c, AURORA_ERR_INVALID, "image width and height must be positive");
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    const size_t bytes = (size_t)cam->image_width * cam->image_height * 12;
+    DevBuf b_rgb, b_a, b_n;
+    Status r = b_rgb.alloc(bytes);
+    if (!r.bad() && aov) r = b_a.alloc(bytes);
+    if (!r.bad() && aov) r = b_n.alloc(bytes);
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(b_rgb.p, rgb, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+    if (!r.bad() && aov) r = hip_status(hipMemcpyAsync(b_a.p, alb, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+    if (!r.bad() && aov) r = hip_status(hipMemcpyAsync(b_n.p, nrm, bytes, hipMemcpyHostToDevice, c->stream), "copy");
+    if (!r.bad())
+        r = render_device(s, cam, rd, b_rgb.as<float>(), b_a.as<float>(), b_n.as<float>(), stats,
+                          (rd->flags & AURORA_RENDER_COUNT_FETCHES) != 0);
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(rgb, b_rgb.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r.bad() && aov) r = hip_status(hipMemcpyAsync(alb, b_a.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r.bad() && aov) r = hip_status(hipMemcpyAsync(nrm, b_n.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r.bad()) r = hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    (void)hipStreamSynchronize(c->stream);
+    b_rgb.release();
+    b_a.release();
+    b_n.release();
+    if (r.bad()) return fail(c, r);
+    return AURORA_OK;
+}
+
+int aurora_resolve_rgb8(const float* rgb, uint32_t w, uint32_t h, uint32_t spp, uint8_t* out) {
+    if (!rgb || !out || spp == 0) return fail(nullptr, AURORA_ERR_INVALID, "NULL buffer or spp == 0");
+    const float scale = 1.0f / (float)spp;
+    const size_t n = (size_t)w * h * 3;
+    for (size_t i = 0; i < n; ++i) {
+        float x = rgb[i] * scale;
+        x = x > 0.0f ? sqrtf(x) : 0.0f;  // gamma 2
+        x = std::fmin(std::fmax(x, 0.0f), 0.999f);
+        out[i] = (uint8_t)(256.0f * x);
+    }
+    return AURORA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// internal.hpp — libaurora internals shared by the host-side C ABI and the
+// kernel translation units.  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "dev_common.hpp"
+
+namespace aur {
+
+struct Status {
+    int code = AURORA_OK;
+    std::string msg;
+    static Status ok() { return Status(); }
+    bool bad() const { return code != AURORA_OK; }
+};
+
+#define AUR_HIP(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return Status{e_ == hipErrorOutOfMemory ? AURORA_ERR_NOMEM : AURORA_ERR_RUNTIME,      \
+                          std::string(#expr) + ": " + hipGetErrorString(e_)};                      \
+    } while (0)
+
+#define AUR_TRY(expr)             \
+    do {                          \
+        Status s_ = (expr);       \
+        if (s_.bad()) return s_;  \
+    } while (0)
+
+// Device buffer owned by the library.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    Status alloc(size_t n) {
+        if (n <= bytes && p) return Status::ok();
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (n == 0) return Status::ok();
+        AUR_HIP(hipMalloc(&p, n));
+        bytes = n;
+        return Status::ok();
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Scratch of the LBVH build.
+struct BvhScratch {
+    DevBuf box_lo, box_hi;      // float4 per prim (padded boxes)
+    DevBuf bounds;              // 12 ordered-u32: scene lo/hi, centroid lo/hi
+    DevBuf keys[2], vals[2];    // radix sort ping-pong
+    DevBuf hist;                // 256 * blocks
+    DevBuf gamma, flags, int_parent, leaf_parent, int_pos, leaf_pos, arrive, depth;
+};
+
+// Build the LBVH into `nodes` (2n float4 pairs) on `stream`.  morton/order
+// receive the sorted codes and primitive order.
+Status lbvh_build(const DevScene& sc, uint32_t n_prims, DevBuf& nodes, DevBuf& morton, DevBuf& order,
+                  BvhScratch& scr, hipStream_t stream, aurora_bvh_stats* stats);
+
+// Extend (closest hit) launch parameters.
+struct ExtendArgs {
+    const float4* ray_o;      // (origin, any)
+    const float4* ray_d;      // (direction, any)
+    const uint32_t* count;    // device count of rays (or nullptr: use n)
+    uint32_t n;
+    uint32_t* pool;           // work counter, must be 0 at launch
+    uint32_t* zero_after;     // optional: written 0 by the kernel (next queue count)
+    float tmin, tmax;
+    float2* hit;              // (t, prim bits)
+    uint32_t* counts;         // optional: 3 per ray (nodes, spheres, quads fetched)
+    unsigned long long* totals; // optional: [nodes, spheres, quads] summed
+    uint32_t* spill;          // stack spill, spill_stride * extend_spill_levels() u32
+    uint32_t spill_stride;    // threads in the persistent grid
+};
+Status launch_extend(const DevScene& sc, const ExtendArgs& a, int grid, hipStream_t stream);
+Status launch_brute(const DevScene& sc, const float4* o, const float4* d, uint32_t n, float tmin, float tmax,
+                    float2* hit, hipStream_t stream);
+int extend_block();         // threads per block of the extend kernel
+int extend_spill_levels();  // u32 stack entries per thread in ExtendArgs::spill
+
+// Wavefront render.
+struct CamF {
+    v3 center, p00, du, dv, disk_u, disk_v;
+    uint32_t W, H;
+    uint32_t defocus;
+};
+struct RenderArgs {
+    CamF cam;
+    uint32_t seed, sample_base, spp_pass, max_depth;
+    float tmin;
+    v3 bg;
+    bool aov;
+};
+struct Workspace {
+    DevBuf q_o[2], q_d[2], q_b[2];   // path queues (float4 each)
+    DevBuf hit;                       // float2 per slot
+    DevBuf L;                         // float4 per path id (radiance)
+    DevBuf aov_a, aov_n;              // float4 per path id
+    DevBuf counters;                  // [count0, count1, pool, pad...]
+    DevBuf spill;
+    uint32_t capacity = 0;
+    uint32_t* host_counts = nullptr;  // pinned, 64 slots
+};
+Status launch_raygen(const DevScene& sc, const RenderArgs& r, Workspace& ws, uint32_t n_paths, hipStream_t s);
+Status launch_shade(const DevScene& sc, const RenderArgs& r, Workspace& ws, int cur, uint32_t depth, int grid,
+                    hipStream_t s);
+Status launch_accumulate(const RenderArgs& r, Workspace& ws, float* rgb, float* alb, float* nrm, hipStream_t s);
+
+}  // namespace aur

@@ -1,0 +1,432 @@
+// lbvh.hip — LBVH build on gfx950.
+//
+// Pipeline (one stream, no host round trips):
+//   k_bounds_init -> k_prim_bounds (padded primitive boxes + wave-reduced
+//   scene/centroid bounds, ordered-u32 atomics) -> k_morton (30-bit codes)
+//   -> 4 x {k_hist, k_scan, k_scatter} (stable LSD radix sort of (code, id)
+//   with 8-bit digits; the scatter ranks keys with wave64 ballots and LDS
+//   per-wave digit counts) -> k_karras (Karras 2012 radix tree) -> k_refit
+//   (bottom-up box union, one acquire/release arrival counter per node).
+//
+// Layout: node 0 is the root, node 1 padding; the children of the internal
+// node with split gamma sit at 2+2*gamma and 3+2*gamma, so every child pair
+// is one aligned 64-byte line.  Each node is two float4: (lo, child) and
+// (hi, parent).
+//
+// No reference counterpart: the reference scans every triangle for every ray
+// (experiments/rt.cpp:209-218).
+#include "internal.hpp"
+
+namespace aur {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;
+constexpr uint32_t kNone = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t f2o(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float o2f(uint32_t o) {
+    union {
+        uint32_t u;
+        float f;
+    } c;
+    c.u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return c.f;
+}
+
+__device__ __forceinline__ void prim_box(const DevScene& sc, uint32_t id, v3& lo, v3& hi) {
+    if (id < sc.n_spheres) {
+        float4 s = sc.spheres[id];
+        lo = V(s.x - s.w, s.y - s.w, s.z - s.w);
+        hi = V(s.x + s.w, s.y + s.w, s.z + s.w);
+    } else {
+        const float4* q = sc.quads + 3 * (size_t)(id - sc.n_spheres);
+        float4 q0 = q[0], q1 = q[1], q2 = q[2];
+        const bool para = __float_as_uint(q1.w) == AURORA_QUAD_PARALLELOGRAM;
+        float a[3] = {q0.x, q0.y, q0.z}, u[3] = {q1.x, q1.y, q1.z}, v[3] = {q2.x, q2.y, q2.z};
+        float l[3], h[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float p0 = a[k], p1 = a[k] + u[k], p2 = a[k] + v[k];
+            float lk = fminf(p0, fminf(p1, p2)), hk = fmaxf(p0, fmaxf(p1, p2));
+            if (para) {
+                float p3 = (a[k] + u[k]) + v[k];
+                lk = fminf(lk, p3);
+                hk = fmaxf(hk, p3);
+            }
+            l[k] = lk;
+            h[k] = hk;
+        }
+        lo = V(l[0], l[1], l[2]);
+        hi = V(h[0], h[1], h[2]);
+    }
+    // pad by 2^-16 of the largest coordinate magnitude (planar quads get
+    // thickness; slab-test rounding cannot cull a true hit)
+    float m = fmaxf(fmaxf(fabsf(lo.x), fabsf(hi.x)), 0.0f);
+    m = fmaxf(m, fmaxf(fabsf(lo.y), fabsf(hi.y)));
+    m = fmaxf(m, fmaxf(fabsf(lo.z), fabsf(hi.z)));
+    float e = m * (1.0f / 65536.0f) + 1e-30f;
+    lo = V(lo.x - e, lo.y - e, lo.z - e);
+    hi = V(hi.x + e, hi.y + e, hi.z + e);
+}
+
+__global__ void k_bounds_init(uint32_t* b) {
+    int t = threadIdx.x;
+    if (t < 12) b[t] = ((t / 3) % 2 == 0) ? 0xffffffffu : 0u;  // lo: min, hi: max
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// bounds: [0..2] scene lo, [3..5] scene hi, [6..8] centroid lo, [9..11] hi
+__global__ __launch_bounds__(kBlock) void k_prim_bounds(DevScene sc, uint32_t n, float4* box_lo, float4* box_hi,
+                                                        uint32_t* bounds) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    float r[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) r[k] = ((k / 3) % 2 == 0) ? INFINITY : -INFINITY;
+    if (i < n) {
+        v3 lo, hi;
+        prim_box(sc, i, lo, hi);
+        box_lo[i] = make_float4(lo.x, lo.y, lo.z, 0.0f);
+        box_hi[i] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+        float l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float c = (l[k] + h[k]) * 0.5f;
+            r[k] = l[k];
+            r[3 + k] = h[k];
+            r[6 + k] = c;
+            r[9 + k] = c;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) r[k] = ((k / 3) % 2 == 0) ? wave_min(r[k]) : wave_max(r[k]);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            if (((k / 3) % 2) == 0) atomicMin(&bounds[k], f2o(r[k]));
+            else atomicMax(&bounds[k], f2o(r[k]));
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t quantise(float c, float lo, float hi) {
+    float e = hi - lo;
+    float x = e > 0.0f ? (c - lo) / e : 0.0f;
+    float q = fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f);
+    return (uint32_t)q;
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton(uint32_t n, const float4* box_lo, const float4* box_hi,
+                                                   const uint32_t* bounds, uint32_t* keys, uint32_t* vals) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float4 lo = box_lo[i], hi = box_hi[i];
+    float cx = (lo.x + hi.x) * 0.5f, cy = (lo.y + hi.y) * 0.5f, cz = (lo.z + hi.z) * 0.5f;
+    uint32_t qx = quantise(cx, o2f(bounds[6]), o2f(bounds[9]));
+    uint32_t qy = quantise(cy, o2f(bounds[7]), o2f(bounds[10]));
+    uint32_t qz = quantise(cz, o2f(bounds[8]), o2f(bounds[11]));
+    keys[i] = (expand10(qx) << 2) | (expand10(qy) << 1) | expand10(qz);
+    vals[i] = i;
+}
+
+// ---- radix sort: per-tile digit histograms (digit-major layout) ----
+__global__ __launch_bounds__(kBlock) void k_hist(const uint32_t* keys, uint32_t n, int shift, uint32_t* hist,
+                                                 uint32_t nblocks) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    size_t base = (size_t)blockIdx.x * kTile;
+#pragma unroll
+    for (int it = 0; it < kItems; ++it) {
+        size_t idx = base + (size_t)it * kBlock + threadIdx.x;
+        if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// exclusive scan of `total` counts by one block of 1024 threads
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* h, uint32_t total) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t chunk = (total + 1023u) / 1024u;
+    const uint32_t b = t * chunk, e = min(total, b + chunk);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += h[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;  // exclusive prefix of this chunk
+    for (uint32_t i = b; i < e; ++i) {
+        uint32_t c = h[i];
+        h[i] = run;
+        run += c;
+    }
+}
+
+// stable scatter: ranks within a wave from 8 ballots over the digit bits,
+// across the 4 waves of a 256-key chunk through LDS counts
+__global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                    uint32_t* vout, uint32_t n, int shift, const uint32_t* hist,
+                                                    uint32_t nblocks) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    run[tid] = hist[(size_t)tid * nblocks + blockIdx.x];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    size_t base = (size_t)blockIdx.x * kTile;
+    for (int it = 0; it < kItems; ++it) {
+        size_t idx = base + (size_t)it * kBlock + tid;
+        bool act = idx < n;
+        uint32_t key = act ? kin[idx] : 0u, val = act ? vin[idx] : 0u;
+        uint32_t dg = (key >> shift) & 255u;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) wcnt[w][tid] = 0;
+        __syncthreads();
+        uint64_t peers = __ballot(act);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            bool bit = (dg >> b) & 1u;
+            uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t rank = __popcll(peers & lt);
+        uint32_t tot = __popcll(peers);
+        if (act && rank == tot - 1) wcnt[wave][dg] = tot;
+        __syncthreads();
+        {
+            uint32_t r = run[tid];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                uint32_t c = wcnt[w][tid];
+                wcnt[w][tid] = r;
+                r += c;
+            }
+            run[tid] = r;
+        }
+        __syncthreads();
+        if (act) {
+            uint32_t dst = wcnt[wave][dg] + rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int delta(const uint32_t* k, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = ((uint64_t)k[i] << 32) | (uint64_t)i;
+    uint64_t b = ((uint64_t)k[j] << 32) | (uint64_t)j;
+    return __clzll(a ^ b);
+}
+
+__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, uint32_t n, uint32_t* gamma,
+                                                   uint32_t* flags, uint32_t* int_parent, uint32_t* leaf_parent,
+                                                   uint32_t* int_pos, uint32_t* leaf_pos, float4* nodes) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t N = n;
+    if (i >= N - 1) return;
+    if (i == 0) {
+        int_pos[0] = 0;
+        int_parent[0] = kNone;
+        nodes[2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));  // node 1: padding
+        nodes[3] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(kNone));
+    }
+    const int d = (delta(keys, N, i, i + 1) - delta(keys, N, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, N, i, i - d);
+    int64_t lmax = 2;
+    while (delta(keys, N, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (delta(keys, N, i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = delta(keys, N, i, j);
+    int64_t s = 0, t = l;
+    do {
+        t = (t + 1) / 2;
+        if (delta(keys, N, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int64_t g = i + s * d + (d < 0 ? -1 : 0);
+    const bool ll = (i < j ? i : j) == g;
+    const bool rl = (i > j ? i : j) == g + 1;
+    gamma[i] = (uint32_t)g;
+    flags[i] = (ll ? 1u : 0u) | (rl ? 2u : 0u);
+    const uint32_t ug = (uint32_t)g, ui = (uint32_t)i;
+    if (ll) { leaf_parent[ug] = ui; leaf_pos[ug] = 2 + 2 * ug; }
+    else { int_parent[ug] = ui; int_pos[ug] = 2 + 2 * ug; }
+    if (rl) { leaf_parent[ug + 1] = ui; leaf_pos[ug + 1] = 3 + 2 * ug; }
+    else { int_parent[ug + 1] = ui; int_pos[ug + 1] = 3 + 2 * ug; }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(uint32_t* p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, const uint32_t* order, const float4* box_lo,
+                                                  const float4* box_hi, const uint32_t* leaf_pos,
+                                                  const uint32_t* leaf_parent, const uint32_t* int_pos,
+                                                  const uint32_t* int_parent, const uint32_t* gamma,
+                                                  uint32_t* arrive, float4* nodes) {
+    const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= n) return;
+    const uint32_t prim = order[m];
+    const float4 blo = box_lo[prim], bhi = box_hi[prim];
+    if (n == 1) {
+        nodes[0] = make_float4(blo.x, blo.y, blo.z, __uint_as_float(AURORA_NODE_LEAF | prim));
+        nodes[1] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(kNone));
+        return;
+    }
+    uint32_t par = leaf_parent[m];
+    const uint32_t pos = leaf_pos[m];
+    nodes[2 * pos] = make_float4(blo.x, blo.y, blo.z, __uint_as_float(AURORA_NODE_LEAF | prim));
+    nodes[2 * pos + 1] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(int_pos[par]));
+    for (;;) {
+        // release this thread's node writes; the second arrival acquires both
+        uint32_t old = __hip_atomic_fetch_add(&arrive[par], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;
+        const uint32_t c = 2 + 2 * gamma[par];
+        const float4 a0 = nodes[2 * c], a1 = nodes[2 * c + 1], b0 = nodes[2 * c + 2], b1 = nodes[2 * c + 3];
+        const uint32_t ppos = int_pos[par];
+        const uint32_t gp = par == 0 ? kNone : int_pos[int_parent[par]];
+        nodes[2 * ppos] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), __uint_as_float(c));
+        nodes[2 * ppos + 1] =
+            make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), __uint_as_float(gp));
+        if (par == 0) return;
+        par = int_parent[par];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_depth(uint32_t n, const uint32_t* leaf_parent,
+                                                  const uint32_t* int_parent, uint32_t* depth) {
+    const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= n || n < 2) return;
+    uint32_t d = 1, p = leaf_parent[m];
+    while (p != 0 && p < n - 1 && d < 256) {
+        p = int_parent[p];
+        ++d;
+    }
+    atomicMax(depth, d);
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+}  // namespace
+
+Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& morton, DevBuf& order,
+                  BvhScratch& s, hipStream_t st, aurora_bvh_stats* stats) {
+    DevScene sc = sc_in;
+    aurora_bvh_stats local{};
+    local.n_prims = n;
+    if (n == 0) {
+        if (stats) *stats = local;
+        return Status::ok();
+    }
+    const uint32_t n_nodes = n == 1 ? 1u : 2u * n;
+    const uint32_t nb = blocks_for(n, kTile);
+    AUR_TRY(nodes.alloc((size_t)n_nodes * 32));
+    AUR_TRY(s.box_lo.alloc((size_t)n * 16));
+    AUR_TRY(s.box_hi.alloc((size_t)n * 16));
+    AUR_TRY(s.bounds.alloc(64));
+    for (int k = 0; k < 2; ++k) {
+        AUR_TRY(s.keys[k].alloc((size_t)n * 4));
+        AUR_TRY(s.vals[k].alloc((size_t)n * 4));
+    }
+    AUR_TRY(s.hist.alloc((size_t)256 * nb * 4));
+    const size_t ni = n > 1 ? n - 1 : 1;
+    AUR_TRY(s.gamma.alloc(ni * 4));
+    AUR_TRY(s.flags.alloc(ni * 4));
+    AUR_TRY(s.int_parent.alloc(ni * 4));
+    AUR_TRY(s.int_pos.alloc(ni * 4));
+    AUR_TRY(s.arrive.alloc(ni * 4));
+    AUR_TRY(s.leaf_parent.alloc((size_t)n * 4));
+    AUR_TRY(s.leaf_pos.alloc((size_t)n * 4));
+    AUR_TRY(s.depth.alloc(4));
+
+    hipEvent_t e0, e1;
+    AUR_HIP(hipEventCreate(&e0));
+    AUR_HIP(hipEventCreate(&e1));
+    AUR_HIP(hipEventRecord(e0, st));
+    k_bounds_init<<<1, 64, 0, st>>>(s.bounds.as<uint32_t>());
+    k_prim_bounds<<<blocks_for(n, kBlock), kBlock, 0, st>>>(sc, n, s.box_lo.as<float4>(), s.box_hi.as<float4>(),
+                                                            s.bounds.as<uint32_t>());
+    k_morton<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.box_lo.as<float4>(), s.box_hi.as<float4>(),
+                                                       s.bounds.as<uint32_t>(), s.keys[0].as<uint32_t>(),
+                                                       s.vals[0].as<uint32_t>());
+    for (int pass = 0; pass < 4; ++pass) {
+        const int a = pass & 1, b = a ^ 1;
+        k_hist<<<nb, kBlock, 0, st>>>(s.keys[a].as<uint32_t>(), n, 8 * pass, s.hist.as<uint32_t>(), nb);
+        k_scan<<<1, 1024, 0, st>>>(s.hist.as<uint32_t>(), 256u * nb);
+        k_scatter<<<nb, kBlock, 0, st>>>(s.keys[a].as<uint32_t>(), s.vals[a].as<uint32_t>(), s.keys[b].as<uint32_t>(),
+                                         s.vals[b].as<uint32_t>(), n, 8 * pass, s.hist.as<uint32_t>(), nb);
+    }
+    // four passes: sorted data is back in buffer 0
+    if (n > 1) {
+        k_karras<<<blocks_for(n - 1, kBlock), kBlock, 0, st>>>(
+            s.keys[0].as<uint32_t>(), n, s.gamma.as<uint32_t>(), s.flags.as<uint32_t>(), s.int_parent.as<uint32_t>(),
+            s.leaf_parent.as<uint32_t>(), s.int_pos.as<uint32_t>(), s.leaf_pos.as<uint32_t>(), nodes.as<float4>());
+        k_fill<<<blocks_for(n - 1, kBlock), kBlock, 0, st>>>(s.arrive.as<uint32_t>(), n - 1, 0u);
+    }
+    k_refit<<<blocks_for(n, kBlock), kBlock, 0, st>>>(
+        n, s.vals[0].as<uint32_t>(), s.box_lo.as<float4>(), s.box_hi.as<float4>(), s.leaf_pos.as<uint32_t>(),
+        s.leaf_parent.as<uint32_t>(), s.int_pos.as<uint32_t>(), s.int_parent.as<uint32_t>(), s.gamma.as<uint32_t>(),
+        s.arrive.as<uint32_t>(), nodes.as<float4>());
+    k_fill<<<1, kBlock, 0, st>>>(s.depth.as<uint32_t>(), 1, 0u);
+    k_depth<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.leaf_parent.as<uint32_t>(), s.int_parent.as<uint32_t>(),
+                                                      s.depth.as<uint32_t>());
+    AUR_HIP(hipGetLastError());
+    AUR_HIP(hipEventRecord(e1, st));
+    std::swap(morton, s.keys[0]);
+    std::swap(order, s.vals[0]);
+
+    uint32_t b[12], depth = 0;
+    AUR_HIP(hipMemcpyAsync(b, s.bounds.p, 48, hipMemcpyDeviceToHost, st));
+    AUR_HIP(hipMemcpyAsync(&depth, s.depth.p, 4, hipMemcpyDeviceToHost, st));
+    AUR_HIP(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    AUR_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    local.n_nodes = n_nodes;
+    local.depth = depth;
+    for (int k = 0; k < 3; ++k) {
+        local.scene_lo[k] = o2f(b[k]);
+        local.scene_hi[k] = o2f(b[3 + k]);
+        local.centroid_lo[k] = o2f(b[6 + k]);
+        local.centroid_hi[k] = o2f(b[9 + k]);
+    }
+    local.build_ms = ms;
+    if (stats) *stats = local;
+    return Status::ok();
+}
+
+}  // namespace aur

@@ -1,0 +1,280 @@
+// render.hip — wavefront path tracing on gfx950: raygen, shade (+ queue
+// compaction) and accumulation.  Extension (closest hit) is k_extend in
+// trace.hip.
+//
+// A pass renders S samples of every pixel as W*H*S paths.  Path ids are
+// sample-major (id = s*W*H + pixel) so the final accumulation reads each
+// sample plane coalesced and adds samples in order, exactly as a sequential
+// per-pixel loop would.  Path state lives in HBM as structure-of-arrays
+// float4 queues (origin+id, direction, throughput); the only radiance a path
+// ever gains comes at its last segment (a light or the background), so the
+// shade kernel writes it once to L[id] and carries no radiance in the queue.
+//
+// Reference counterparts: the per-pixel loop of experiments/rt.cpp:337-374
+// (camera basis :339-343, pixel mapping :364-366) and the bounce loop of
+// rt.cpp:251-334; reflect/refract from src/basic/vec3.cpp:182-199.
+#include "internal.hpp"
+
+namespace aur {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void k_raygen(RenderArgs r, uint32_t n, float4* qo, float4* qd, float4* qb,
+                                                   float4* L, uint32_t* counters) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p == 0) {
+        counters[0] = n;   // queue 0 count
+        counters[1] = 0u;  // queue 1 count
+        counters[2] = 0u;  // extend work pool
+    }
+    if (p >= n) return;
+    const uint32_t WH = r.cam.W * r.cam.H;
+    const uint32_t s_local = p / WH;
+    const uint32_t pixel = p - s_local * WH;
+    const uint32_t i = pixel % r.cam.W, j = pixel / r.cam.W;
+    const uint32_t sample = r.sample_base + s_local;
+    const u4 q = philox(u4{pixel, sample, 0u, 0u}, r.seed, STREAM_RENDER);
+    const float ox = u01(q.x) - 0.5f, oy = u01(q.y) - 0.5f;
+    const v3 ps = (r.cam.p00 + ((float)i + ox) * r.cam.du) + ((float)j + oy) * r.cam.dv;
+    v3 org = r.cam.center;
+    if (r.cam.defocus) {
+        float px = 0.0f, py = 0.0f;
+        for (uint32_t k = 0; k < 32; ++k) {
+            const u4 c = philox(u4{pixel, sample, 0u, 1u + k}, r.seed, STREAM_RENDER);
+            const float ax = 2.0f * u01(c.x) - 1.0f, ay = 2.0f * u01(c.y) - 1.0f;
+            if (ax * ax + ay * ay < 1.0f) {
+                px = ax;
+                py = ay;
+                break;
+            }
+        }
+        org = (org + px * r.cam.disk_u) + py * r.cam.disk_v;
+    }
+    const v3 dir = unitv(ps - org);
+    qo[p] = make_float4(org.x, org.y, org.z, __uint_as_float(p));
+    qd[p] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    qb[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ v3 random_unit(uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t seed) {
+    for (uint32_t k = 0; k < 64; ++k) {
+        const u4 c = philox(u4{pixel, sample, bounce, k}, seed, STREAM_RENDER);
+        const v3 p = V(2.0f * u01(c.x) - 1.0f, 2.0f * u01(c.y) - 1.0f, 2.0f * u01(c.z) - 1.0f);
+        const float l2 = dot(p, p);
+        if (l2 > 1e-30f && l2 <= 1.0f) return p / sqrtf(l2);
+    }
+    return V(0.0f, 0.0f, 1.0f);
+}
+
+__device__ __forceinline__ float schlick(float cosine, float ri) {
+    float r0 = (1.0f - ri) / (1.0f + ri);
+    r0 = r0 * r0;
+    const float x = 1.0f - cosine;
+    const float x5 = ((x * x) * (x * x)) * x;
+    return r0 + (1.0f - r0) * x5;
+}
+
+struct ShadeBufs {
+    const float4 *qo, *qd, *qb;
+    const float2* hit;
+    float4 *no, *nd, *nb;
+    float4* L;
+    float4 *aov_a, *aov_n;
+    uint32_t* counters;
+    int cur;
+};
+
+// grid-stride over the live queue; survivors are appended to the next queue
+// with one atomic per wave (ballot + prefix popcount)
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, RenderArgs r, ShadeBufs b, uint32_t depth) {
+    const uint32_t n = b.counters[b.cur];
+    uint32_t* next_count = &b.counters[b.cur ^ 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) b.counters[2] = 0u;  // pool of the next extend
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t WH = r.cam.W * r.cam.H;
+    const bool aov = r.aov && depth == 0;
+    const v3 bg = r.bg;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool alive = false;
+        v3 p = V(0, 0, 0), nd = V(0, 0, 0), beta = V(0, 0, 0);
+        uint32_t path = 0;
+        if (i < n) {
+            const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
+            const float2 H = b.hit[i];
+            path = __float_as_uint(O.w);
+            const v3 org = xyz(O), dir = xyz(D);
+            beta = xyz(B);
+            const int prim = __float_as_int(H.y);
+            const uint32_t s_local = path / WH;
+            const uint32_t pixel = path - s_local * WH;
+            const uint32_t sample = r.sample_base + s_local;
+            if (prim < 0) {
+                const v3 e = beta * bg;
+                b.L[path] = make_float4(e.x, e.y, e.z, 0.0f);
+                if (aov) {
+                    const v3 a = clamp01(bg);
+                    b.aov_a[path] = make_float4(a.x, a.y, a.z, 0.0f);
+                    b.aov_n[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+            } else {
+                const uint32_t id = (uint32_t)prim;
+                const float t = H.x;
+                p = org + t * dir;
+                v3 outward;
+                uint32_t mi;
+                const bool is_sphere = id < sc.n_spheres;
+                float4 q0, q1, q2;
+                if (is_sphere) {
+                    const float4 s = sc.spheres[id];
+                    outward = (p - xyz(s)) / s.w;
+                    mi = sc.sphere_mat[id];
+                } else {
+                    const float4* q = sc.quads + 3 * (size_t)(id - sc.n_spheres);
+                    q0 = q[0];
+                    q1 = q[1];
+                    q2 = q[2];
+                    outward = unitv(cross(xyz(q1), xyz(q2)));
+                    mi = __float_as_uint(q0.w);
+                }
+                const aurora_material m = sc.mats[mi];
+                float tu = 0.0f, tv = 0.0f;
+                if (m.type != AURORA_MAT_DIELECTRIC && tex_needs_uv(sc, m.texture)) {
+                    if (is_sphere) {
+                        sphere_uv(outward, tu, tv);
+                    } else {
+                        float tt;
+                        hit_quad(q0, q1, q2, org, dir, r.tmin, INFINITY, tt, tu, tv);
+                    }
+                }
+                const bool front = dot(dir, outward) < 0.0f;
+                const v3 n = front ? outward : -outward;
+                v3 att = V(1.0f, 1.0f, 1.0f);
+                bool scatter = true;
+                if (m.type == AURORA_MAT_DIFFUSE_LIGHT) {
+                    const v3 e0 = texture_value(sc, m.texture, tu, tv, p);
+                    const v3 e = beta * e0;
+                    b.L[path] = make_float4(e.x, e.y, e.z, 0.0f);
+                    att = clamp01(e0);
+                    scatter = false;
+                } else if (m.type == AURORA_MAT_LAMBERTIAN) {
+                    att = texture_value(sc, m.texture, tu, tv, p);
+                    v3 sd = n + random_unit(pixel, sample, depth + 1, r.seed);
+                    if (near_zero(sd)) sd = n;
+                    nd = unitv(sd);
+                } else if (m.type == AURORA_MAT_METAL) {
+                    att = texture_value(sc, m.texture, tu, tv, p);
+                    v3 rf = unitv(reflect(dir, n));
+                    if (m.fuzz > 0.0f) rf = rf + m.fuzz * random_unit(pixel, sample, depth + 1, r.seed);
+                    if (dot(rf, n) <= 0.0f) scatter = false;  // absorbed
+                    else nd = unitv(rf);
+                } else {
+                    const float ri = front ? 1.0f / m.ior : m.ior;
+                    const float cos_t = fminf(dot(-dir, n), 1.0f);
+                    const float sin_t = sqrtf(1.0f - cos_t * cos_t);
+                    const bool cannot = ri * sin_t > 1.0f;
+                    const u4 c = philox(u4{pixel, sample, depth + 1, 0u}, r.seed, STREAM_RENDER);
+                    const float uu = u01(c.x);
+                    const v3 d2 = (cannot || schlick(cos_t, ri) > uu) ? reflect(dir, n) : refract(dir, n, ri);
+                    nd = unitv(d2);
+                }
+                if (aov) {
+                    const v3 a = clamp01(att);
+                    b.aov_a[path] = make_float4(a.x, a.y, a.z, 0.0f);
+                    b.aov_n[path] = make_float4(n.x, n.y, n.z, 0.0f);
+                }
+                if (scatter) {
+                    beta = beta * att;
+                    alive = depth + 1 < r.max_depth;
+                }
+            }
+        }
+        const uint64_t m = __ballot(alive);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t slot0 = 0;
+            if (lane == leader) slot0 = atomicAdd(next_count, (uint32_t)__popcll(m));
+            slot0 = __shfl(slot0, leader);
+            if (alive) {
+                const uint32_t slot = slot0 + (uint32_t)__popcll(m & lt);
+                b.no[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(path));
+                b.nd[slot] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+                b.nb[slot] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+            }
+        }
+    }
+}
+
+// per pixel: add the pass's samples in sample order to the caller's sums
+__global__ __launch_bounds__(kBlock) void k_accumulate(uint32_t WH, uint32_t S, const float4* L, const float4* A,
+                                                       const float4* Nn, float* rgb, float* alb, float* nrm) {
+    const uint32_t px = blockIdx.x * kBlock + threadIdx.x;
+    if (px >= WH) return;
+    v3 acc = V(rgb[3 * (size_t)px], rgb[3 * (size_t)px + 1], rgb[3 * (size_t)px + 2]);
+    for (uint32_t s = 0; s < S; ++s) acc = acc + xyz(L[(size_t)s * WH + px]);
+    rgb[3 * (size_t)px] = acc.x;
+    rgb[3 * (size_t)px + 1] = acc.y;
+    rgb[3 * (size_t)px + 2] = acc.z;
+    if (alb) {
+        v3 a = V(alb[3 * (size_t)px], alb[3 * (size_t)px + 1], alb[3 * (size_t)px + 2]);
+        v3 q = V(nrm[3 * (size_t)px], nrm[3 * (size_t)px + 1], nrm[3 * (size_t)px + 2]);
+        for (uint32_t s = 0; s < S; ++s) {
+            a = a + xyz(A[(size_t)s * WH + px]);
+            q = q + xyz(Nn[(size_t)s * WH + px]);
+        }
+        alb[3 * (size_t)px] = a.x;
+        alb[3 * (size_t)px + 1] = a.y;
+        alb[3 * (size_t)px + 2] = a.z;
+        nrm[3 * (size_t)px] = q.x;
+        nrm[3 * (size_t)px + 1] = q.y;
+        nrm[3 * (size_t)px + 2] = q.z;
+    }
+}
+
+}  // namespace
+
+Status launch_raygen(const DevScene& sc, const RenderArgs& r, Workspace& ws, uint32_t n, hipStream_t s) {
+    (void)sc;
+    const uint32_t blocks = (n + kBlock - 1) / kBlock;
+    k_raygen<<<blocks ? blocks : 1, kBlock, 0, s>>>(r, n, ws.q_o[0].as<float4>(), ws.q_d[0].as<float4>(),
+                                                    ws.q_b[0].as<float4>(), ws.L.as<float4>(),
+                                                    ws.counters.as<uint32_t>());
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status launch_shade(const DevScene& sc, const RenderArgs& r, Workspace& ws, int cur, uint32_t depth, int grid,
+                    hipStream_t s) {
+    ShadeBufs b;
+    b.qo = ws.q_o[cur].as<float4>();
+    b.qd = ws.q_d[cur].as<float4>();
+    b.qb = ws.q_b[cur].as<float4>();
+    b.hit = ws.hit.as<float2>();
+    b.no = ws.q_o[cur ^ 1].as<float4>();
+    b.nd = ws.q_d[cur ^ 1].as<float4>();
+    b.nb = ws.q_b[cur ^ 1].as<float4>();
+    b.L = ws.L.as<float4>();
+    b.aov_a = ws.aov_a.as<float4>();
+    b.aov_n = ws.aov_n.as<float4>();
+    b.counters = ws.counters.as<uint32_t>();
+    b.cur = cur;
+    k_shade<<<grid, kBlock, 0, s>>>(sc, r, b, depth);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status launch_accumulate(const RenderArgs& r, Workspace& ws, float* rgb, float* alb, float* nrm, hipStream_t s) {
+    const uint32_t WH = r.cam.W * r.cam.H;
+    k_accumulate<<<(WH + kBlock - 1) / kBlock, kBlock, 0, s>>>(WH, r.spp_pass, ws.L.as<float4>(),
+                                                               ws.aov_a.as<float4>(), ws.aov_n.as<float4>(), rgb,
+                                                               r.aov ? alb : nullptr, r.aov ? nrm : nullptr);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+}  // namespace aur

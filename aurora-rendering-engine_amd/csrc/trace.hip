@@ -1,0 +1,240 @@
+// trace.hip — closest-hit traversal of the LBVH on gfx950 (the extend phase
+// of the wavefront path tracer and the parity entry aurora_trace_closest).
+//
+// k_extend is persistent: a grid sized to the resident capacity of the chip
+// pulls rays from a global pool.  Each lane owns one ray; when enough lanes
+// of a wave64 have finished, the wave claims that many new rays with one
+// atomic and hands them out by ballot prefix (refilling idle lanes).  One
+// step fetches a 64-byte aligned child pair (both boxes), tests both slabs,
+// intersects leaf primitives at once and descends to the nearer child,
+// pushing the farther.  The stack lives in LDS (kLds entries per lane, laid
+// out [level][thread] so a wave's pushes hit 64 consecutive banks) and
+// spills deeper levels to a per-thread global array; the Karras tree depth
+// is below 64 because every level lengthens the 64-bit key prefix.
+//
+// Reference counterpart: the per-ray linear scan over all triangles,
+// experiments/rt.cpp:209-218, with are::Triangle::intersect_ray
+// (src/object/triangle.cpp:82-121) as the primitive test.
+#include "internal.hpp"
+
+namespace aur {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kLds = 16;
+constexpr int kMaxStack = 64;
+constexpr int kRefill = 16;
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr uint32_t kNoPrim = 0x7fffffffu;
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, ExtendArgs a) {
+    __shared__ uint32_t stk[kLds][kBlock];
+    const uint32_t n = a.count ? *a.count : a.n;
+    if (a.zero_after && blockIdx.x == 0 && threadIdx.x == 0) *a.zero_after = 0u;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t* spill = a.spill + (size_t)blockIdx.x * kBlock + tid;
+    const size_t sstride = a.spill_stride;
+    const float4* N = sc.nodes;
+    const float tmin = a.tmin, tmax = a.tmax;
+
+    uint32_t ri = kNone;
+    bool done = false;
+    v3 o = V(0, 0, 0), d = V(0, 0, 0), inv = V(0, 0, 0);
+    float best_t = tmax;
+    uint32_t best_id = kNoPrim;
+    uint32_t cur = 0;
+    int sp = 0;
+    uint32_t cn = 0, cs = 0, cq = 0;                  // per-ray counts
+    unsigned long long tn = 0, ts = 0, tq = 0;        // per-lane totals
+
+    const uint32_t n_prims = sc.n_spheres + sc.n_quads;
+    auto test_prim = [&](uint32_t id) {
+        float t;
+        bool h;
+        if (id >= n_prims) return;
+        if (id < sc.n_spheres) {
+            if (COUNT) ++cs;
+            h = hit_sphere(sc.spheres[id], o, d, tmin, tmax, t);
+        } else {
+            if (COUNT) ++cq;
+            const float4* q = sc.quads + 3 * (size_t)(id - sc.n_spheres);
+            float al, be;
+            h = hit_quad(q[0], q[1], q[2], o, d, tmin, tmax, t, al, be);
+        }
+        if (h && (t < best_t || (t == best_t && id < best_id))) {
+            best_t = t;
+            best_id = id;
+        }
+    };
+    auto finish = [&]() {
+        const bool miss = best_id == kNoPrim;
+        a.hit[ri] = make_float2(miss ? tmax : best_t, __int_as_float(miss ? AURORA_MISS : (int)best_id));
+        if (COUNT) {
+            if (a.counts) {
+                a.counts[3 * (size_t)ri] = cn;
+                a.counts[3 * (size_t)ri + 1] = cs;
+                a.counts[3 * (size_t)ri + 2] = cq;
+            }
+            tn += cn;
+            ts += cs;
+            tq += cq;
+        }
+        ri = kNone;
+    };
+
+    for (;;) {
+        const bool idle = (ri == kNone) && !done;
+        const uint64_t im = __ballot(idle);
+        const uint64_t am = __ballot(ri != kNone);
+        if (im != 0 && (am == 0 || __popcll(im) >= kRefill)) {
+            const uint32_t cnt = (uint32_t)__popcll(im);
+            const int leader = __ffsll((long long)im) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(a.pool, cnt);
+            base = __shfl(base, leader);
+            if (idle) {
+                const uint32_t my = base + (uint32_t)__popcll(im & lt);
+                if (my >= n) {
+                    done = true;
+                } else {
+                    ri = my;
+                    const float4 ro = a.ray_o[ri], rd = a.ray_d[ri];
+                    o = xyz(ro);
+                    d = xyz(rd);
+                    inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                    best_t = tmax;
+                    best_id = kNoPrim;
+                    sp = 0;
+                    cn = cs = cq = 0;
+                    if (sc.n_nodes == 0) {
+                        finish();
+                    } else {
+                        const float4 r0 = N[0], r1 = N[1];
+                        if (COUNT) ++cn;
+                        float tr;
+                        const uint32_t rc = __float_as_uint(r0.w);
+                        if (!slab(r0, r1, o, inv, tmin, best_t, tr)) {
+                            finish();
+                        } else if (rc & AURORA_NODE_LEAF) {
+                            test_prim(rc & ~AURORA_NODE_LEAF);
+                            finish();
+                        } else {
+                            cur = rc;
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot((ri != kNone) || !done) == 0) break;
+        if (ri == kNone) continue;
+
+        // a well-formed tree only holds even pair indices below n_nodes; any
+        // other value ends the ray instead of reading out of bounds
+        if ((cur & 1u) || cur + 1 >= sc.n_nodes) {
+            finish();
+            continue;
+        }
+        // one traversal step: the child pair at nodes[cur], nodes[cur+1]
+        const float4 a0 = N[2 * (size_t)cur], a1 = N[2 * (size_t)cur + 1];
+        const float4 b0 = N[2 * (size_t)cur + 2], b1 = N[2 * (size_t)cur + 3];
+        if (COUNT) cn += 2;
+        float ta, tb;
+        bool ha = slab(a0, a1, o, inv, tmin, best_t, ta);
+        bool hb = slab(b0, b1, o, inv, tmin, best_t, tb);
+        const uint32_t ca = __float_as_uint(a0.w), cb = __float_as_uint(b0.w);
+        if (ha && (ca & AURORA_NODE_LEAF)) {
+            test_prim(ca & ~AURORA_NODE_LEAF);
+            ha = false;
+        }
+        if (hb && (cb & AURORA_NODE_LEAF)) {
+            test_prim(cb & ~AURORA_NODE_LEAF);
+            hb = false;
+        }
+        if (ha && hb) {
+            const bool bnear = tb < ta;
+            const uint32_t far_ = bnear ? ca : cb;
+            cur = bnear ? cb : ca;
+            // the tree is shallower than kMaxStack; the bound only keeps every
+            // access inside the stack arrays
+            if (sp < kLds) stk[sp][tid] = far_;
+            else if (sp < kMaxStack) spill[(size_t)(sp - kLds) * sstride] = far_;
+            if (sp < kMaxStack) ++sp;
+        } else if (ha) {
+            cur = ca;
+        } else if (hb) {
+            cur = cb;
+        } else if (sp == 0) {
+            finish();
+        } else {
+            --sp;
+            cur = sp < kLds ? stk[sp][tid] : spill[(size_t)(sp - kLds) * sstride];
+        }
+    }
+    if (COUNT && a.totals) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            tn += __shfl_xor(tn, off);
+            ts += __shfl_xor(ts, off);
+            tq += __shfl_xor(tq, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.totals[0], tn);
+            atomicAdd(&a.totals[1], ts);
+            atomicAdd(&a.totals[2], tq);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_brute(DevScene sc, const float4* ro, const float4* rd, uint32_t n,
+                                                  float tmin, float tmax, float2* hit) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const v3 o = xyz(ro[i]), d = xyz(rd[i]);
+    float best_t = tmax;
+    uint32_t best_id = kNoPrim;
+    const uint32_t np = sc.n_spheres + sc.n_quads;
+    for (uint32_t id = 0; id < np; ++id) {
+        float t;
+        bool h;
+        if (id < sc.n_spheres) {
+            h = hit_sphere(sc.spheres[id], o, d, tmin, tmax, t);
+        } else {
+            const float4* q = sc.quads + 3 * (size_t)(id - sc.n_spheres);
+            float al, be;
+            h = hit_quad(q[0], q[1], q[2], o, d, tmin, tmax, t, al, be);
+        }
+        if (h && (t < best_t || (t == best_t && id < best_id))) {
+            best_t = t;
+            best_id = id;
+        }
+    }
+    const bool miss = best_id == kNoPrim;
+    hit[i] = make_float2(miss ? tmax : best_t, __int_as_float(miss ? AURORA_MISS : (int)best_id));
+}
+
+}  // namespace
+
+int extend_block() { return kBlock; }
+int extend_spill_levels() { return kMaxStack - kLds; }
+
+Status launch_extend(const DevScene& sc, const ExtendArgs& a, int grid, hipStream_t s) {
+    if ((size_t)grid * kBlock > a.spill_stride) return Status{AURORA_ERR_RUNTIME, "extend grid exceeds spill stride"};
+    if (a.counts || a.totals) k_extend<true><<<grid, kBlock, 0, s>>>(sc, a);
+    else k_extend<false><<<grid, kBlock, 0, s>>>(sc, a);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status launch_brute(const DevScene& sc, const float4* o, const float4* d, uint32_t n, float tmin, float tmax,
+                    float2* hit, hipStream_t s) {
+    if (n == 0) return Status::ok();
+    k_brute<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(sc, o, d, n, tmin, tmax, hit);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+}  // namespace aur

@@ -1,0 +1,267 @@
+/*
+ * aurora.h — C ABI of libaurora, the MI355X (gfx950) path-tracing hot path.
+ *
+ * Every entry point takes plain pointers and sizes, returns a status code and
+ * never throws.  The host C++ API (namespace are: hittable / material /
+ * texture / camera) flattens a scene into the descriptors below and calls
+ * these functions; ctypes, cgo or JNI bindings can do the same.
+ *
+ * The reference renderer (ARE) has no path tracer, BVH, sphere, quad,
+ * scatter or camera; each entry point names the reference code whose role
+ * it takes over:
+ *   - scene objects      : are::Object / are::Triangle / are::ObjectSet
+ *                          (include/object/object.h:14-39,
+ *                           include/object/triangle.h:19,
+ *                           include/object/object_set.h:10-12)
+ *   - ray query          : are::Triangle::intersect_ray
+ *                          (src/object/triangle.cpp:82-121) and the linear
+ *                          closest-hit scan experiments/rt.cpp:209-218
+ *   - per-pixel render   : experiments/rt.cpp:337-374 (render) and
+ *                          experiments/rt.cpp:251-334 (trace)
+ *   - image output       : are::Texture::save_texture (src/texture.cpp:362-395)
+ *
+ * Conventions
+ *   - status 0 = AURORA_OK.  On failure, aurora_last_error(ctx) holds a
+ *     message; the host C++ wrapper rethrows it as std::invalid_argument
+ *     (AURORA_ERR_INVALID) or std::runtime_error (everything else), the
+ *     exception types the reference's constructors throw
+ *     (src/object/triangle.cpp:13-36, src/texture.cpp:11-43).
+ *   - the caller owns host buffers; the library owns device buffers behind
+ *     the opaque handles.  Entry points with a _device suffix take device
+ *     pointers that the caller allocated on the context's device.
+ *   - one HIP stream per context; a context is not thread-safe, separate
+ *     contexts are independent.
+ */
+#ifndef AURORA_H
+#define AURORA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AURORA_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+#define AURORA_OK 0
+#define AURORA_ERR_INVALID 1   /* bad argument or scene descriptor        */
+#define AURORA_ERR_RUNTIME 2   /* HIP runtime / kernel launch failure      */
+#define AURORA_ERR_NOMEM 3     /* device allocation failed                 */
+#define AURORA_ERR_STATE 4     /* call out of order (e.g. no BVH built)    */
+
+/* ---- primitive kinds -------------------------------------------------- */
+/* A quad record (Q, u, v) spans the parallelogram Q + a*u + b*v with
+ * 0<=a,b<=1; the triangle kind keeps a,b>=0, a+b<=1 (vertices Q, Q+u, Q+v as
+ * in are::Triangle, src/object/triangle.cpp:42-45). */
+#define AURORA_QUAD_PARALLELOGRAM 0u
+#define AURORA_QUAD_TRIANGLE 1u
+
+/* ---- materials -------------------------------------------------------- */
+#define AURORA_MAT_LAMBERTIAN 0u     /* albedo = texture                    */
+#define AURORA_MAT_METAL 1u          /* albedo = texture, fuzz              */
+#define AURORA_MAT_DIELECTRIC 2u     /* ior                                 */
+#define AURORA_MAT_DIFFUSE_LIGHT 3u  /* emit = texture, no scatter          */
+
+typedef struct aurora_material {
+    uint32_t type;     /* AURORA_MAT_*                                      */
+    uint32_t texture;  /* index into the texture table (unused: dielectric) */
+    float fuzz;        /* metal: clamped to [0,1] at upload                  */
+    float ior;         /* dielectric: refraction index                       */
+} aurora_material;
+
+/* ---- textures --------------------------------------------------------- */
+#define AURORA_TEX_SOLID 0u    /* color                                     */
+#define AURORA_TEX_CHECKER 1u  /* spatial checker: even/odd texture, scale  */
+#define AURORA_TEX_NOISE 2u    /* Perlin marble: scale, seed                */
+#define AURORA_TEX_IMAGE 3u    /* bilinear RGB image: width, height, offset */
+
+typedef struct aurora_texture {
+    uint32_t type;       /* AURORA_TEX_*                                    */
+    uint32_t even, odd;  /* checker: texture indices (must not be checkers) */
+    uint32_t seed;       /* noise: Philox seed of the Perlin tables         */
+    float color[3];      /* solid                                           */
+    float scale;         /* checker: cell size; noise: frequency            */
+    uint32_t width, height;   /* image                                      */
+    uint64_t texel_offset;    /* image: first texel (RGB float triple) in
+                                 aurora_scene_desc.texels                   */
+} aurora_texture;
+
+/* ---- scene ------------------------------------------------------------ */
+typedef struct aurora_scene_desc {
+    uint32_t n_spheres;
+    const float* spheres;           /* n_spheres x {cx, cy, cz, radius}     */
+    const uint32_t* sphere_material;/* n_spheres                            */
+    uint32_t n_quads;
+    const float* quads;             /* n_quads x {Qx,Qy,Qz, ux,uy,uz, vx,vy,vz} */
+    const uint32_t* quad_material;  /* n_quads                              */
+    const uint32_t* quad_kind;      /* n_quads, AURORA_QUAD_* (NULL = all
+                                       parallelograms)                      */
+    uint32_t n_materials;
+    const aurora_material* materials;
+    uint32_t n_textures;
+    const aurora_texture* textures;
+    uint64_t n_texels;              /* RGB triples                          */
+    const float* texels;            /* 3*n_texels floats, linear RGB        */
+} aurora_scene_desc;
+
+/* Primitive ids used by every query: spheres are [0, n_spheres), quads are
+ * [n_spheres, n_spheres + n_quads).  AURORA_MISS marks no hit. */
+#define AURORA_MISS (-1)
+
+/* ---- LBVH ------------------------------------------------------------- */
+/* 32-byte node.  Node 0 is the root and node 1 is unused padding; every
+ * internal node's two children sit side by side at nodes[child] and
+ * nodes[child+1] with `child` even, so one 64-byte-aligned fetch reads both
+ * child boxes.  The tree of n primitives has 2n nodes (n = 1: one leaf).
+ * A leaf has AURORA_NODE_LEAF set in `child` and holds the primitive id in
+ * the low 31 bits. */
+#define AURORA_NODE_LEAF 0x80000000u
+typedef struct aurora_bvh_node {
+    float lo[3];
+    uint32_t child;
+    float hi[3];
+    uint32_t parent;  /* index of the parent node (root: 0xffffffff)     */
+} aurora_bvh_node;
+
+typedef struct aurora_bvh_stats {
+    uint32_t n_prims;
+    uint32_t n_nodes;       /* 2*n_prims (1 for a single primitive)         */
+    uint32_t depth;         /* longest root-to-leaf path, in edges          */
+    uint32_t reserved;
+    float scene_lo[3], scene_hi[3];     /* union of primitive boxes         */
+    float centroid_lo[3], centroid_hi[3];/* Morton quantisation frame        */
+    double build_ms;        /* device time of the build kernels            */
+} aurora_bvh_stats;
+
+/* ---- camera and render ------------------------------------------------ */
+typedef struct aurora_camera_desc {
+    uint32_t image_width, image_height;
+    float vfov;            /* vertical field of view, degrees              */
+    float lookfrom[3], lookat[3], vup[3];
+    float defocus_angle;   /* degrees; <= 0 disables depth of field        */
+    float focus_dist;
+} aurora_camera_desc;
+
+#define AURORA_RENDER_AOV 1u   /* also write first-hit albedo and normal */
+#define AURORA_RENDER_COUNT_FETCHES 2u  /* count BVH node and primitive
+                                           records the traversal fetches
+                                           (slower; for roofline accounting) */
+
+typedef struct aurora_render_desc {
+    uint32_t spp;            /* samples this call adds                       */
+    uint32_t sample_offset;  /* first sample index (progressive rendering)   */
+    uint32_t max_depth;      /* ray segments per path (>= 1)                 */
+    uint32_t seed;           /* Philox key word 0                            */
+    float background[3];     /* radiance of rays that leave the scene        */
+    uint32_t flags;          /* AURORA_RENDER_*                              */
+    uint32_t spp_per_pass;   /* samples per wavefront pass (0 = automatic)   */
+    float tmin;              /* ray offset; <= 0 selects 0.001               */
+} aurora_render_desc;
+
+typedef struct aurora_render_stats {
+    uint64_t paths;          /* camera samples traced                        */
+    uint64_t segments;       /* ray segments traversed (extension rays)      */
+    double kernel_ms;        /* device time from first to last render kernel */
+    double extend_ms;        /* summed device time of the traversal launches */
+    uint32_t passes;
+    uint32_t max_live;       /* largest wavefront queue                      */
+    uint32_t extend_launches;
+    uint32_t reserved;
+    uint64_t nodes_fetched;  /* AURORA_RENDER_COUNT_FETCHES: 32-byte nodes   */
+    uint64_t spheres_fetched;/*   16-byte sphere records                     */
+    uint64_t quads_fetched;  /*   48-byte quad records                       */
+} aurora_render_stats;
+
+/* ---- context ---------------------------------------------------------- */
+typedef struct aurora_ctx aurora_ctx;
+typedef struct aurora_scene aurora_scene;
+
+int aurora_abi_version(void);
+
+/* Create a context on HIP device `device` (takes over from nothing in the
+ * reference: ARE is single-threaded CPU code without a device). */
+int aurora_create(int device, aurora_ctx** out);
+void aurora_destroy(aurora_ctx* ctx);
+/* Message of the last failure on ctx (ctx NULL: last failure of
+ * aurora_create on this thread).  Never NULL. */
+const char* aurora_last_error(const aurora_ctx* ctx);
+/* Synchronise the context's stream. */
+int aurora_synchronize(aurora_ctx* ctx);
+/* The context's HIP stream, as a void* (hipStream_t). */
+void* aurora_stream(aurora_ctx* ctx);
+/* Persistent grid sizes of the traversal and shade kernels, in blocks per
+ * compute unit (default 8 each; both in [1, 64]).  Changes speed only. */
+int aurora_set_launch(aurora_ctx* ctx, int extend_blocks_per_cu,
+                      int shade_blocks_per_cu);
+
+/* Validate and upload a flattened scene (replaces are::ObjectSet,
+ * include/object/object_set.h:10-12, with its Triangle, Material and Texture
+ * pointers, include/object/triangle.h:38-39).  Rejects degenerate quads the
+ * way are::Triangle's constructor does (src/object/triangle.cpp:25-36). */
+int aurora_scene_upload(aurora_ctx* ctx, const aurora_scene_desc* desc,
+                        aurora_scene** out);
+void aurora_scene_destroy(aurora_scene* scene);
+
+/* Build the LBVH on the device: Morton codes, LDS radix sort, Karras radix
+ * tree, bottom-up refit into 32-byte nodes.  No reference counterpart (ARE
+ * scans linearly, experiments/rt.cpp:209-218). */
+int aurora_bvh_build(aurora_scene* scene, aurora_bvh_stats* stats);
+/* Copy the node array (stats.n_nodes nodes) to host memory. */
+int aurora_bvh_download(aurora_scene* scene, aurora_bvh_node* nodes,
+                        uint32_t capacity);
+/* Copy the sorted 30-bit Morton codes and primitive order to host memory. */
+int aurora_bvh_download_order(aurora_scene* scene, uint32_t* morton,
+                              uint32_t* prim_order, uint32_t capacity);
+
+/* Closest hit of n rays in (tmin, tmax): the role of
+ * are::Triangle::intersect_ray (src/object/triangle.cpp:82-121) over the
+ * linear scan of experiments/rt.cpp:209-218, on spheres and quads through
+ * the LBVH.  Directions need not be unit length; t is in units of the given
+ * direction.  Ties in t go to the lower primitive id.  origins/dirs are
+ * 3*n floats; prim receives the primitive id or AURORA_MISS, t the hit
+ * distance (tmax on a miss).  Host buffers. */
+int aurora_trace_closest(aurora_scene* scene, const float* origins,
+                         const float* dirs, uint32_t n, float tmin,
+                         float tmax, int32_t* prim, float* t);
+/* Same on device buffers; counts (optional, device, 2*n u32) receives the
+ * number of BVH nodes and primitive records fetched per ray. */
+int aurora_trace_closest_device(aurora_scene* scene, const float* d_origins,
+                                const float* d_dirs, uint32_t n, float tmin,
+                                float tmax, int32_t* d_prim, float* d_t,
+                                uint32_t* d_counts);
+/* Brute-force closest hit (every primitive tested, no BVH): the semantics
+ * of experiments/rt.cpp:209-218.  Parity aid. Device buffers. */
+int aurora_trace_brute_device(aurora_scene* scene, const float* d_origins,
+                              const float* d_dirs, uint32_t n, float tmin,
+                              float tmax, int32_t* d_prim, float* d_t);
+
+/* Render: the role of experiments/rt.cpp:337-374 (per-pixel loop) and
+ * rt.cpp:251-334 (trace).  Adds the radiance of samples
+ * [sample_offset, sample_offset+spp) of every pixel to rgb_sum (W*H*3
+ * floats, row 0 = top; caller zeroes it before the first call).  With
+ * AURORA_RENDER_AOV, albedo and normal (W*H*3 each) receive the sum over the
+ * same samples of the first hit's albedo and shading normal.  Host buffers;
+ * aovs may be NULL. */
+int aurora_render(aurora_scene* scene, const aurora_camera_desc* cam,
+                  const aurora_render_desc* rd, float* rgb_sum,
+                  float* albedo_sum, float* normal_sum,
+                  aurora_render_stats* stats);
+/* Same on device buffers (no PCIe traffic). */
+int aurora_render_device(aurora_scene* scene, const aurora_camera_desc* cam,
+                         const aurora_render_desc* rd, float* d_rgb_sum,
+                         float* d_albedo_sum, float* d_normal_sum,
+                         aurora_render_stats* stats);
+
+/* Resolve a radiance sum to 8-bit sRGB-ish bytes: x = clamp(sqrt(sum/spp),
+ * 0, 0.999) * 256, truncated; byte layout of a P6 PPM body
+ * (src/texture.cpp:378-391).  Host buffers. */
+int aurora_resolve_rgb8(const float* rgb_sum, uint32_t width,
+                        uint32_t height, uint32_t spp, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AURORA_H */

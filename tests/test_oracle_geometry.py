@@ -1,0 +1,143 @@
+"""Closed-form pins of the oracle's primitive tests and scatter helpers."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import orc
+
+
+def test_sphere_through_centre():
+    # t = |oc| - r for a ray aimed at the centre
+    ok, t = orc.hit_sphere([0, 0, 5, 2], [0, 0, 0], [0, 0, 1])
+    assert ok and t == 3.0
+    ok, t = orc.hit_sphere([3, 4, 0, 1], [0, 0, 0], [0.6, 0.8, 0])
+    assert ok and abs(t - 4.0) < 1e-6
+
+
+def test_sphere_inside_takes_far_root_and_interval():
+    ok, t = orc.hit_sphere([0, 0, 0, 2], [0, 0, 0], [1, 0, 0])
+    assert ok and t == 2.0
+    ok, _ = orc.hit_sphere([0, 0, 5, 2], [0, 0, 0], [0, 0, 1], tmin=0.0, tmax=2.9)
+    assert not ok
+    ok, t = orc.hit_sphere([0, 0, 5, 2], [0, 0, 0], [0, 0, 1], tmin=3.5)
+    assert ok and t == 7.0  # near root rejected by tmin: far root
+
+
+def test_sphere_miss_and_tangent():
+    assert not orc.hit_sphere([0, 3, 5, 2], [0, 0, 0], [0, 0, 1])[0]
+    ok, t = orc.hit_sphere([0, 2, 5, 2], [0, 0, 0], [0, 0, 1])
+    assert ok and abs(t - 5.0) < 1e-3  # tangent: double root at z = 5
+
+
+def test_unnormalised_direction_scales_t():
+    ok, t = orc.hit_sphere([0, 0, 5, 2], [0, 0, 0], [0, 0, 2])
+    assert ok and t == 1.5
+
+
+QUAD = [0, 0, 0, 2, 0, 0, 0, 3, 0]  # Q, u, v in the z=0 plane
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_quad_interior_and_barycentrics(kind):
+    ok, tab = orc.hit_quad(QUAD, kind, [0.5, 0.6, -4], [0, 0, 1])
+    assert ok and tab[0] == 4.0
+    assert abs(tab[1] - 0.25) < 1e-7 and abs(tab[2] - 0.2) < 1e-7
+
+
+def test_quad_vs_triangle_region():
+    # (1.6, 2.4): alpha 0.8, beta 0.8 -> inside parallelogram, outside triangle
+    assert orc.hit_quad(QUAD, 0, [1.6, 2.4, 1], [0, 0, -1])[0]
+    assert not orc.hit_quad(QUAD, 1, [1.6, 2.4, 1], [0, 0, -1])[0]
+
+
+def test_quad_edges_inclusive_like_reference_triangle():
+    # are::Triangle::intersect_ray accepts alpha/beta on the boundary
+    # (src/object/triangle.cpp:98-111); here with zero tolerance.
+    for p in ([0, 0], [2, 0], [0, 3], [1, 0], [0, 1.5]):
+        assert orc.hit_quad(QUAD, 1, [p[0], p[1], 1], [0, 0, -1])[0], p
+    assert not orc.hit_quad(QUAD, 1, [-1e-3, 1, 1], [0, 0, -1])[0]
+
+
+def test_quad_parallel_and_behind():
+    assert not orc.hit_quad(QUAD, 0, [0.5, 0.5, 1], [1, 0, 0])[0]
+    assert not orc.hit_quad(QUAD, 0, [0.5, 0.5, 1], [0, 0, 1])[0]
+
+
+def test_quad_t_equals_plane_distance():
+    # plane z = 0: t = -o.z / d.z for any oblique ray through the interior
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        p = np.array([rng.uniform(0.1, 1.9), rng.uniform(0.1, 2.9), 0.0])
+        o = p + rng.uniform(-2, 2, 3) * [1, 1, 0] + [0, 0, rng.uniform(0.5, 5)]
+        d = (p - o).astype(np.float32)
+        ok, tab = orc.hit_quad(QUAD, 0, o.astype(np.float32), d)
+        assert ok
+        assert abs(tab[0] - 1.0) < 1e-5
+
+
+def test_reflect_closed_form():
+    v = np.array([1.0, -1.0, 0.0], np.float32) / np.float32(math.sqrt(2))
+    r = orc.reflect(v, [0, 1, 0])
+    assert np.array_equal(r, np.array([v[0], -v[1], 0], np.float32))
+
+
+@pytest.mark.parametrize("eta", [1 / 1.5, 1.5, 1.0, 1 / 1.33])
+def test_refract_snell(eta):
+    n = np.array([0, 1, 0], np.float32)
+    for th in np.linspace(0.05, 1.4, 20):
+        sin_i = math.sin(th)
+        if eta * sin_i >= 1.0:
+            continue
+        v = np.array([sin_i, -math.cos(th), 0], np.float32)
+        r = orc.refract(v, n, eta)
+        assert abs(np.linalg.norm(r) - 1) < 1e-5
+        # Snell: sin(theta_t) = eta sin(theta_i); refracted ray goes down
+        assert abs(r[0] - eta * sin_i) < 1e-5 and r[1] < 0
+
+
+def test_refract_eta_one_is_identity():
+    v = np.array([0.6, -0.8, 0], np.float32)
+    assert np.allclose(orc.refract(v, [0, 1, 0], 1.0), v, atol=1e-6)
+
+
+def test_schlick():
+    r0 = ((1 - 1.5) / (1 + 1.5)) ** 2
+    assert abs(orc.schlick(1.0, 1.5) - r0) < 1e-7  # normal incidence
+    assert abs(orc.schlick(0.0, 1.5) - 1.0) < 1e-6  # grazing
+    c = 0.3
+    assert abs(orc.schlick(c, 1.5) - (r0 + (1 - r0) * (1 - c) ** 5)) < 1e-6
+
+
+def test_random_unit_is_unit_and_isotropic():
+    v = np.array([orc.random_unit(p, 0, 1) for p in range(4000)], np.float64)
+    assert np.allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-6)
+    # isotropy: mean ~ 0, second moment ~ I/3 (4000 samples: se ~ 0.009)
+    assert np.all(np.abs(v.mean(0)) < 0.05)
+    m2 = v.T @ v / len(v)
+    assert np.allclose(m2, np.eye(3) / 3, atol=0.04)
+
+
+def test_lambertian_cosine_distribution_chi2():
+    # n + random_unit, normalised, is cosine-distributed: cos(theta) = sqrt(U)
+    # => cos^2 uniform on [0,1].  Chi-square over 10 bins, 5000 samples.
+    n = np.array([0, 0, 1.0])
+    c2 = []
+    for p in range(5000):
+        d = n + orc.random_unit(p, 3, 2).astype(np.float64)
+        d /= np.linalg.norm(d)
+        c2.append(d[2] ** 2)
+    h, _ = np.histogram(c2, bins=10, range=(0, 1))
+    e = len(c2) / 10
+    chi2 = float(((h - e) ** 2 / e).sum())
+    assert chi2 < 27.9  # p = 0.001 for 9 dof
+
+
+def test_sin_atan_acos_vs_libm():
+    xs = np.concatenate([np.linspace(-50, 50, 2001), [0.0, 1e-8, 1234.5, -987.25]])
+    for x in xs:
+        assert abs(orc.lib().orc_sin(x) - math.sin(x)) < 1e-12
+    for y, x in [(1, 1), (-1, 1), (1, -1), (-1, -1), (0, -1), (0.3, 2.0), (5.0, 0.01), (0, 0)]:
+        assert abs(orc.lib().orc_atan2(y, x) - math.atan2(y, x)) < 1e-14
+    for y in np.linspace(-1, 1, 101):
+        assert abs(orc.lib().orc_acos(y) - math.acos(y)) < 1e-7
