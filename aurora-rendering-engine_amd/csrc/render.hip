@@ -87,126 +87,165 @@ struct ShadeBufs {
     int cur;
 };
 
-// grid-stride over the live queue; survivors are appended to the next queue
-// with one atomic per wave (ballot + prefix popcount)
+struct ShadeOut {
+    v3 p, nd, beta;
+    uint32_t path;
+    bool alive;
+};
+
+// one path: emission or background on termination, else scatter
+__device__ __forceinline__ ShadeOut shade_one(const DevScene& sc, const RenderArgs& r, const ShadeBufs& b,
+                                              uint32_t i, uint32_t depth, bool aov, uint32_t WH) {
+    ShadeOut out;
+    out.alive = false;
+    const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
+    const float2 H = b.hit[i];
+    const uint32_t path = __float_as_uint(O.w);
+    out.path = path;
+    const v3 org = xyz(O), dir = xyz(D);
+    v3 beta = xyz(B);
+    const int prim = __float_as_int(H.y);
+    const uint32_t s_local = path / WH;
+    const uint32_t pixel = path - s_local * WH;
+    const uint32_t sample = r.sample_base + s_local;
+    if (prim < 0) {
+        const v3 e = beta * r.bg;
+        b.L[path] = make_float4(e.x, e.y, e.z, 0.0f);
+        if (aov) {
+            const v3 a = clamp01(r.bg);
+            b.aov_a[path] = make_float4(a.x, a.y, a.z, 0.0f);
+            b.aov_n[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        return out;
+    }
+    const uint32_t id = (uint32_t)prim;
+    const v3 p = org + H.x * dir;
+    v3 outward;
+    uint32_t mi;
+    const bool is_sphere = id < sc.n_spheres;
+    float4 q0, q1, q2;
+    if (is_sphere) {
+        const float4 s = sc.spheres[id];
+        outward = (p - xyz(s)) / s.w;
+        mi = sc.sphere_mat[id];
+    } else {
+        const float4* q = sc.quads + 3 * (size_t)(id - sc.n_spheres);
+        q0 = q[0];
+        q1 = q[1];
+        q2 = q[2];
+        outward = unitv(cross(xyz(q1), xyz(q2)));
+        mi = __float_as_uint(q0.w);
+    }
+    const aurora_material m = sc.mats[mi];
+    float tu = 0.0f, tv = 0.0f;
+    if (m.type != AURORA_MAT_DIELECTRIC && tex_needs_uv(sc, m.texture)) {
+        if (is_sphere) {
+            sphere_uv(outward, tu, tv);
+        } else {
+            float tt;
+            hit_quad(q0, q1, q2, org, dir, r.tmin, INFINITY, tt, tu, tv);
+        }
+    }
+    const bool front = dot(dir, outward) < 0.0f;
+    const v3 n = front ? outward : -outward;
+    v3 att = V(1.0f, 1.0f, 1.0f);
+    v3 nd = V(0.0f, 0.0f, 0.0f);
+    bool scatter = true;
+    if (m.type == AURORA_MAT_DIFFUSE_LIGHT) {
+        const v3 e0 = texture_value(sc, m.texture, tu, tv, p);
+        const v3 e = beta * e0;
+        b.L[path] = make_float4(e.x, e.y, e.z, 0.0f);
+        att = clamp01(e0);
+        scatter = false;
+    } else if (m.type == AURORA_MAT_LAMBERTIAN) {
+        att = texture_value(sc, m.texture, tu, tv, p);
+        v3 sd = n + random_unit(pixel, sample, depth + 1, r.seed);
+        if (near_zero(sd)) sd = n;
+        nd = unitv(sd);
+    } else if (m.type == AURORA_MAT_METAL) {
+        att = texture_value(sc, m.texture, tu, tv, p);
+        v3 rf = unitv(reflect(dir, n));
+        if (m.fuzz > 0.0f) rf = rf + m.fuzz * random_unit(pixel, sample, depth + 1, r.seed);
+        if (dot(rf, n) <= 0.0f) scatter = false;  // absorbed
+        else nd = unitv(rf);
+    } else {
+        const float ri = front ? 1.0f / m.ior : m.ior;
+        const float cos_t = fminf(dot(-dir, n), 1.0f);
+        const float sin_t = sqrtf(1.0f - cos_t * cos_t);
+        const bool cannot = ri * sin_t > 1.0f;
+        const u4 c = philox(u4{pixel, sample, depth + 1, 0u}, r.seed, STREAM_RENDER);
+        const float uu = u01(c.x);
+        const v3 d2 = (cannot || schlick(cos_t, ri) > uu) ? reflect(dir, n) : refract(dir, n, ri);
+        nd = unitv(d2);
+    }
+    if (aov) {
+        const v3 a = clamp01(att);
+        b.aov_a[path] = make_float4(a.x, a.y, a.z, 0.0f);
+        b.aov_n[path] = make_float4(n.x, n.y, n.z, 0.0f);
+    }
+    if (scatter) {
+        beta = beta * att;
+        out.alive = depth + 1 < r.max_depth;
+    }
+    out.p = p;
+    out.nd = nd;
+    out.beta = beta;
+    return out;
+}
+
+constexpr int kShadeItems = 4;
+constexpr uint32_t kShadeTile = kBlock * kShadeItems;
+
+// Grid-stride over tiles of 1024 live paths.  Survivors of a tile are
+// appended to the next queue with one atomic per block: counts are scanned
+// within each wave (shuffles) and across the 4 waves (LDS).
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, RenderArgs r, ShadeBufs b, uint32_t depth) {
+    __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint32_t s_base;
     const uint32_t n = b.counters[b.cur];
     uint32_t* next_count = &b.counters[b.cur ^ 1];
     if (blockIdx.x == 0 && threadIdx.x == 0) b.counters[2] = 0u;  // pool of the next extend
-    const int lane = threadIdx.x & 63;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t WH = r.cam.W * r.cam.H;
     const bool aov = r.aov && depth == 0;
-    const v3 bg = r.bg;
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += stride) {
-        const uint32_t i = base + threadIdx.x;
-        bool alive = false;
-        v3 p = V(0, 0, 0), nd = V(0, 0, 0), beta = V(0, 0, 0);
-        uint32_t path = 0;
-        if (i < n) {
-            const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
-            const float2 H = b.hit[i];
-            path = __float_as_uint(O.w);
-            const v3 org = xyz(O), dir = xyz(D);
-            beta = xyz(B);
-            const int prim = __float_as_int(H.y);
-            const uint32_t s_local = path / WH;
-            const uint32_t pixel = path - s_local * WH;
-            const uint32_t sample = r.sample_base + s_local;
-            if (prim < 0) {
-                const v3 e = beta * bg;
-                b.L[path] = make_float4(e.x, e.y, e.z, 0.0f);
-                if (aov) {
-                    const v3 a = clamp01(bg);
-                    b.aov_a[path] = make_float4(a.x, a.y, a.z, 0.0f);
-                    b.aov_n[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                }
-            } else {
-                const uint32_t id = (uint32_t)prim;
-                const float t = H.x;
-                p = org + t * dir;
-                v3 outward;
-                uint32_t mi;
-                const bool is_sphere = id < sc.n_spheres;
-                float4 q0, q1, q2;
-                if (is_sphere) {
-                    const float4 s = sc.spheres[id];
-                    outward = (p - xyz(s)) / s.w;
-                    mi = sc.sphere_mat[id];
-                } else {
-                    const float4* q = sc.quads + 3 * (size_t)(id - sc.n_spheres);
-                    q0 = q[0];
-                    q1 = q[1];
-                    q2 = q[2];
-                    outward = unitv(cross(xyz(q1), xyz(q2)));
-                    mi = __float_as_uint(q0.w);
-                }
-                const aurora_material m = sc.mats[mi];
-                float tu = 0.0f, tv = 0.0f;
-                if (m.type != AURORA_MAT_DIELECTRIC && tex_needs_uv(sc, m.texture)) {
-                    if (is_sphere) {
-                        sphere_uv(outward, tu, tv);
-                    } else {
-                        float tt;
-                        hit_quad(q0, q1, q2, org, dir, r.tmin, INFINITY, tt, tu, tv);
-                    }
-                }
-                const bool front = dot(dir, outward) < 0.0f;
-                const v3 n = front ? outward : -outward;
-                v3 att = V(1.0f, 1.0f, 1.0f);
-                bool scatter = true;
-                if (m.type == AURORA_MAT_DIFFUSE_LIGHT) {
-                    const v3 e0 = texture_value(sc, m.texture, tu, tv, p);
-                    const v3 e = beta * e0;
-                    b.L[path] = make_float4(e.x, e.y, e.z, 0.0f);
-                    att = clamp01(e0);
-                    scatter = false;
-                } else if (m.type == AURORA_MAT_LAMBERTIAN) {
-                    att = texture_value(sc, m.texture, tu, tv, p);
-                    v3 sd = n + random_unit(pixel, sample, depth + 1, r.seed);
-                    if (near_zero(sd)) sd = n;
-                    nd = unitv(sd);
-                } else if (m.type == AURORA_MAT_METAL) {
-                    att = texture_value(sc, m.texture, tu, tv, p);
-                    v3 rf = unitv(reflect(dir, n));
-                    if (m.fuzz > 0.0f) rf = rf + m.fuzz * random_unit(pixel, sample, depth + 1, r.seed);
-                    if (dot(rf, n) <= 0.0f) scatter = false;  // absorbed
-                    else nd = unitv(rf);
-                } else {
-                    const float ri = front ? 1.0f / m.ior : m.ior;
-                    const float cos_t = fminf(dot(-dir, n), 1.0f);
-                    const float sin_t = sqrtf(1.0f - cos_t * cos_t);
-                    const bool cannot = ri * sin_t > 1.0f;
-                    const u4 c = philox(u4{pixel, sample, depth + 1, 0u}, r.seed, STREAM_RENDER);
-                    const float uu = u01(c.x);
-                    const v3 d2 = (cannot || schlick(cos_t, ri) > uu) ? reflect(dir, n) : refract(dir, n, ri);
-                    nd = unitv(d2);
-                }
-                if (aov) {
-                    const v3 a = clamp01(att);
-                    b.aov_a[path] = make_float4(a.x, a.y, a.z, 0.0f);
-                    b.aov_n[path] = make_float4(n.x, n.y, n.z, 0.0f);
-                }
-                if (scatter) {
-                    beta = beta * att;
-                    alive = depth + 1 < r.max_depth;
-                }
+    for (uint32_t tile = blockIdx.x * kShadeTile; tile < n; tile += gridDim.x * kShadeTile) {
+        ShadeOut o[kShadeItems];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < kShadeItems; ++k) {
+            const uint32_t i = tile + k * kBlock + tid;
+            if (i < n) o[k] = shade_one(sc, r, b, i, depth, aov, WH);
+            else o[k].alive = false;
+            cnt += o[k].alive ? 1u : 0u;
+        }
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (lane == 63) s_wave[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) {
+            const uint32_t c = s_wave[w];
+            before += w < wave ? c : 0u;
+            total += c;
+        }
+        if (tid == 0) s_base = total ? atomicAdd(next_count, total) : 0u;
+        __syncthreads();
+        uint32_t slot = s_base + before + incl - cnt;
+#pragma unroll
+        for (int k = 0; k < kShadeItems; ++k) {
+            if (o[k].alive) {
+                b.no[slot] = make_float4(o[k].p.x, o[k].p.y, o[k].p.z, __uint_as_float(o[k].path));
+                b.nd[slot] = make_float4(o[k].nd.x, o[k].nd.y, o[k].nd.z, 0.0f);
+                b.nb[slot] = make_float4(o[k].beta.x, o[k].beta.y, o[k].beta.z, 0.0f);
+                ++slot;
             }
         }
-        const uint64_t m = __ballot(alive);
-        if (m) {
-            const int leader = __ffsll((long long)m) - 1;
-            uint32_t slot0 = 0;
-            if (lane == leader) slot0 = atomicAdd(next_count, (uint32_t)__popcll(m));
-            slot0 = __shfl(slot0, leader);
-            if (alive) {
-                const uint32_t slot = slot0 + (uint32_t)__popcll(m & lt);
-                b.no[slot] = make_float4(p.x, p.y, p.z, __uint_as_float(path));
-                b.nd[slot] = make_float4(nd.x, nd.y, nd.z, 0.0f);
-                b.nb[slot] = make_float4(beta.x, beta.y, beta.z, 0.0f);
-            }
-        }
+        __syncthreads();  // s_wave / s_base reused by the next tile
     }
 }
 

@@ -24,7 +24,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kLds = 16;
 constexpr int kMaxStack = 64;
-constexpr int kRefill = 16;
+constexpr int kRefill = 8;
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kNoPrim = 0x7fffffffu;
 
@@ -86,22 +86,31 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, ExtendArgs a) {
         ri = kNone;
     };
 
+    // Work distribution: a wave claims a batch of rays with one atomic and
+    // deals them out to its idle lanes (ballot rank) until the batch is
+    // spent, so the shared pool counter sees one atomic per batch.
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    const uint32_t batch = min(1024u, max(64u, n / (waves * 4u) + 1u));
+    uint32_t b_next = 0, b_end = 0;  // wave-uniform
     for (;;) {
         const bool idle = (ri == kNone) && !done;
         const uint64_t im = __ballot(idle);
         const uint64_t am = __ballot(ri != kNone);
         if (im != 0 && (am == 0 || __popcll(im) >= kRefill)) {
-            const uint32_t cnt = (uint32_t)__popcll(im);
-            const int leader = __ffsll((long long)im) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(a.pool, cnt);
-            base = __shfl(base, leader);
+            if (b_next >= b_end) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(a.pool, batch);
+                base = __shfl(base, 0);
+                b_next = base;
+                b_end = base < n ? min(n, base + batch) : base;
+            }
+            const uint32_t avail = b_end - b_next;
+            const uint32_t rank = (uint32_t)__popcll(im & lt);
             if (idle) {
-                const uint32_t my = base + (uint32_t)__popcll(im & lt);
-                if (my >= n) {
+                if (avail == 0) {
                     done = true;
-                } else {
-                    ri = my;
+                } else if (rank < avail) {
+                    ri = b_next + rank;
                     const float4 ro = a.ray_o[ri], rd = a.ray_d[ri];
                     o = xyz(ro);
                     d = xyz(rd);
@@ -128,6 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, ExtendArgs a) {
                     }
                 }
             }
+            b_next += min(avail, (uint32_t)__popcll(im));
         }
         if (__ballot((ri != kNone) || !done) == 0) break;
         if (ri == kNone) continue;
