@@ -248,8 +248,8 @@ Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n,
     } else {
         const int grid = grid_for_extend(c, n);
         AUR_TRY(ensure_spill(s->tr_spill, grid));
-        AUR_TRY(s->tr_pool.alloc(16));
-        AUR_HIP(hipMemsetAsync(s->tr_pool.p, 0, 4, c->stream));
+        AUR_TRY(s->tr_pool.alloc(kPools * kPoolStride * 4));
+        AUR_HIP(hipMemsetAsync(s->tr_pool.p, 0, kPools * kPoolStride * 4, c->stream));
         ExtendArgs a{};
         a.ray_o = s->tr_o.as<float4>();
         a.ray_d = s->tr_d.as<float4>();
@@ -283,7 +283,7 @@ Status ensure_workspace(aurora_scene* s, uint32_t cap, int ext_grid) {
         AUR_TRY(w.aov_n.alloc((size_t)cap * 16));
         w.capacity = cap;
     }
-    AUR_TRY(w.counters.alloc(64));
+    AUR_TRY(w.counters.alloc(kCounterBytes));
     AUR_TRY(ensure_spill(w.spill, ext_grid));
     if (!w.host_counts) AUR_HIP(hipHostMalloc((void**)&w.host_counts, 256 * sizeof(uint32_t), hipHostMallocDefault));
     return Status::ok();
@@ -359,12 +359,24 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
         uint32_t launched = 0;
         for (uint32_t depth = 0; depth < rd->max_depth; ++depth) {
             const int cur = depth & 1;
+            // queues only shrink, so the count two bounces back bounds this
+            // bounce's queue: size both launches from it
+            uint32_t ub = n;
+            if (depth >= 2) {
+                AUR_HIP(hipEventSynchronize(cnt_ev[depth - 2]));
+                ub = w.host_counts[depth - 2];
+                if (ub == 0) break;
+            }
+            const int eg = (int)std::min<uint64_t>((uint64_t)ext_grid,
+                                                   std::max<uint64_t>(1, ((uint64_t)ub + 1023) / 1024));
+            const int sg = (int)std::min<uint64_t>((uint64_t)c->shade_grid,
+                                                   std::max<uint64_t>(1, ((uint64_t)ub + 2047) / 2048));
             ExtendArgs a{};
             a.ray_o = w.q_o[cur].as<float4>();
             a.ray_d = w.q_d[cur].as<float4>();
             a.count = w.counters.as<uint32_t>() + cur;
             a.n = 0;
-            a.pool = w.counters.as<uint32_t>() + 2;
+            a.pool = w.counters.as<uint32_t>() + kPoolBase;
             a.zero_after = w.counters.as<uint32_t>() + (cur ^ 1);
             a.tmin = r.tmin;
             a.tmax = INFINITY;
@@ -378,23 +390,17 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
                 AUR_TRY(ev(e1));
                 AUR_HIP(hipEventRecord(e0, c->stream));
             }
-            AUR_TRY(launch_extend(s->view, a, ext_grid, c->stream));
+            AUR_TRY(launch_extend(s->view, a, eg, c->stream));
             if (timed) {
                 AUR_HIP(hipEventRecord(e1, c->stream));
                 ext_ev.push_back(e0);
                 ext_ev.push_back(e1);
             }
-            AUR_TRY(launch_shade(s->view, r, w, cur, depth, c->shade_grid, c->stream));
+            AUR_TRY(launch_shade(s->view, r, w, cur, depth, sg, c->stream));
             AUR_HIP(hipMemcpyAsync(w.host_counts + depth, w.counters.as<uint32_t>() + (cur ^ 1), 4,
                                    hipMemcpyDeviceToHost, c->stream));
             AUR_HIP(hipEventRecord(cnt_ev[depth], c->stream));
             launched = depth + 1;
-            // stop once a queue two bounces back came out empty (keeps two
-            // bounces in flight while the host checks)
-            if (depth >= 2) {
-                AUR_HIP(hipEventSynchronize(cnt_ev[depth - 2]));
-                if (w.host_counts[depth - 2] == 0) break;
-            }
         }
         AUR_TRY(launch_accumulate(r, w, rgb, alb, nrm, c->stream));
         AUR_HIP(hipStreamSynchronize(c->stream));

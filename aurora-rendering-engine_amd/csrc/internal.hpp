@@ -11,6 +11,13 @@
 
 namespace aur {
 
+// extend work pools: one counter per XCD, each on its own 128-byte line
+constexpr int kPools = 8;
+constexpr uint32_t kPoolStride = 32;
+// render counters: [0], [1] queue counts, pools from kPoolBase
+constexpr uint32_t kPoolBase = 32;
+constexpr size_t kCounterBytes = (kPoolBase + kPools * kPoolStride) * 4;
+
 struct Status {
     int code = AURORA_OK;
     std::string msg;
@@ -72,7 +79,7 @@ struct ExtendArgs {
     const float4* ray_d;      // (direction, any)
     const uint32_t* count;    // device count of rays (or nullptr: use n)
     uint32_t n;
-    uint32_t* pool;           // work counter, must be 0 at launch
+    uint32_t* pool;           // kPools work counters kPoolStride u32 apart, 0 at launch
     uint32_t* zero_after;     // optional: written 0 by the kernel (next queue count)
     float tmin, tmax;
     float2* hit;              // (t, prim bits)
@@ -105,7 +112,7 @@ struct Workspace {
     DevBuf hit;                       // float2 per slot
     DevBuf L;                         // float4 per path id (radiance)
     DevBuf aov_a, aov_n;              // float4 per path id
-    DevBuf counters;                  // [count0, count1, pool, pad...]
+    DevBuf counters;                  // queue counts [0], [1]; pools from kPoolBase
     DevBuf spill;
     uint32_t capacity = 0;
     uint32_t* host_counts = nullptr;  // pinned, 64 slots

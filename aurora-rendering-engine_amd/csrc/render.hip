@@ -27,8 +27,8 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderArgs r, uint32_t n, flo
     if (p == 0) {
         counters[0] = n;   // queue 0 count
         counters[1] = 0u;  // queue 1 count
-        counters[2] = 0u;  // extend work pool
     }
+    if (p < kPools) counters[kPoolBase + kPoolStride * p] = 0u;  // extend work pools
     if (p >= n) return;
     const uint32_t WH = r.cam.W * r.cam.H;
     const uint32_t s_local = p / WH;
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, RenderArgs r, Sha
     __shared__ uint32_t s_base;
     const uint32_t n = b.counters[b.cur];
     uint32_t* next_count = &b.counters[b.cur ^ 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) b.counters[2] = 0u;  // pool of the next extend
+    if (blockIdx.x == 0 && threadIdx.x < kPools) b.counters[kPoolBase + kPoolStride * threadIdx.x] = 0u;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t WH = r.cam.W * r.cam.H;
     const bool aov = r.aov && depth == 0;

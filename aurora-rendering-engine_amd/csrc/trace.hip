@@ -25,6 +25,7 @@ constexpr int kBlock = 256;
 constexpr int kLds = 16;
 constexpr int kMaxStack = 64;
 constexpr int kRefill = 8;
+constexpr int kParts = 8;
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kNoPrim = 0x7fffffffu;
 
@@ -86,23 +87,33 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, ExtendArgs a) {
         ri = kNone;
     };
 
-    // Work distribution: a wave claims a batch of rays with one atomic and
-    // deals them out to its idle lanes (ballot rank) until the batch is
-    // spent, so the shared pool counter sees one atomic per batch.
-    const uint32_t waves = gridDim.x * (kBlock / 64);
-    const uint32_t batch = min(1024u, max(64u, n / (waves * 4u) + 1u));
-    uint32_t b_next = 0, b_end = 0;  // wave-uniform
+    // Work distribution: the rays are split into up to kParts contiguous
+    // ranges, one per XCD (blocks are dealt to the 8 XCDs round-robin), each
+    // with its own pool counter on its own 128-byte line.  A wave claims a
+    // batch of its range with one atomic and deals it out to idle lanes by
+    // ballot rank; a plain load skips the atomic once the range is spent.
+    const uint32_t parts = min((uint32_t)kParts, gridDim.x);
+    const uint32_t part = blockIdx.x % parts;
+    const uint32_t r_lo = (uint32_t)(((uint64_t)n * part) / parts);
+    const uint32_t r_n = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - r_lo;
+    uint32_t* pool = a.pool + kPoolStride * part;
+    const uint32_t waves = ((gridDim.x - part + parts - 1) / parts) * (kBlock / 64);
+    const uint32_t batch = min(4096u, max(64u, r_n / (waves * 2u) + 1u));
+    uint32_t b_next = 0, b_end = 0;  // wave-uniform, offsets within the range
     for (;;) {
         const bool idle = (ri == kNone) && !done;
         const uint64_t im = __ballot(idle);
         const uint64_t am = __ballot(ri != kNone);
         if (im != 0 && (am == 0 || __popcll(im) >= kRefill)) {
             if (b_next >= b_end) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(a.pool, batch);
+                uint32_t base = r_n;
+                if (lane == 0) {
+                    if (__hip_atomic_load(pool, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < r_n)
+                        base = atomicAdd(pool, batch);
+                }
                 base = __shfl(base, 0);
                 b_next = base;
-                b_end = base < n ? min(n, base + batch) : base;
+                b_end = base < r_n ? min(r_n, base + batch) : base;
             }
             const uint32_t avail = b_end - b_next;
             const uint32_t rank = (uint32_t)__popcll(im & lt);
@@ -110,7 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, ExtendArgs a) {
                 if (avail == 0) {
                     done = true;
                 } else if (rank < avail) {
-                    ri = b_next + rank;
+                    ri = r_lo + b_next + rank;
                     const float4 ro = a.ray_o[ri], rd = a.ray_d[ri];
                     o = xyz(ro);
                     d = xyz(rd);
