@@ -142,7 +142,7 @@ EXPORTS = [
     ("aurora_last_error", C.c_char_p, [_vp]),
     ("aurora_synchronize", C.c_int, [_vp]),
     ("aurora_stream", _vp, [_vp]),
-    ("aurora_set_launch", C.c_int, [_vp, C.c_int, C.c_int]),
+    ("aurora_set_launch", C.c_int, [_vp, C.c_int, C.c_int, _u32]),
     ("aurora_scene_upload", C.c_int, [_vp, C.POINTER(SceneDesc), C.POINTER(_vp)]),
     ("aurora_scene_destroy", None, [_vp]),
     ("aurora_bvh_build", C.c_int, [_vp, C.POINTER(BvhStats)]),

@@ -162,8 +162,8 @@ class Context:
         except Exception:
             pass
 
-    def set_launch(self, extend_blocks_per_cu, shade_blocks_per_cu):
-        self.check(self.lib.aurora_set_launch(self.h, extend_blocks_per_cu, shade_blocks_per_cu))
+    def set_launch(self, extend_blocks_per_cu=8, shade_blocks_per_cu=8, tail_paths=65536):
+        self.check(self.lib.aurora_set_launch(self.h, extend_blocks_per_cu, shade_blocks_per_cu, tail_paths))
 
     def synchronize(self):
         self.check(self.lib.aurora_synchronize(self.h))

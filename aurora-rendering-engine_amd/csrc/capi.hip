@@ -27,6 +27,7 @@ struct aurora_ctx {
     int cus = 0;
     int ext_grid = 0;    // resident blocks of the extend kernel
     int shade_grid = 0;
+    uint32_t tail_paths = 65536;  // fused-tail switch: live paths at or below
 };
 
 struct aurora_scene {
@@ -355,8 +356,8 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
         st.paths += n;
         st.max_live = std::max(st.max_live, n);
         AUR_TRY(launch_raygen(s->view, r, w, n, c->stream));
-        uint64_t segs = n;
         uint32_t launched = 0;
+        bool tail = false;
         for (uint32_t depth = 0; depth < rd->max_depth; ++depth) {
             const int cur = depth & 1;
             // queues only shrink, so the count two bounces back bounds this
@@ -366,6 +367,12 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
                 AUR_HIP(hipEventSynchronize(cnt_ev[depth - 2]));
                 ub = w.host_counts[depth - 2];
                 if (ub == 0) break;
+            }
+            if (ub <= c->tail_paths) {
+                const int tg = (int)std::min<uint64_t>((uint64_t)ext_grid, ((uint64_t)ub + 255) / 256);
+                AUR_TRY(launch_tail(s->view, r, w, cur, depth, std::max(1, tg), c->stream));
+                tail = true;
+                break;
             }
             const int eg = (int)std::min<uint64_t>((uint64_t)ext_grid,
                                                    std::max<uint64_t>(1, ((uint64_t)ub + 1023) / 1024));
@@ -403,9 +410,17 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
             launched = depth + 1;
         }
         AUR_TRY(launch_accumulate(r, w, rgb, alb, nrm, c->stream));
+        unsigned long long tail_segs = 0;
+        if (tail)
+            AUR_HIP(hipMemcpyAsync(&tail_segs, w.counters.as<uint32_t>() + kTailSegs, 8, hipMemcpyDeviceToHost,
+                                   c->stream));
         AUR_HIP(hipStreamSynchronize(c->stream));
-        for (uint32_t k = 0; k < launched; ++k) segs += w.host_counts[k];
-        st.segments += segs;
+        // queue sizes: bounce 0 holds n paths, bounce j >= 1 host_counts[j-1];
+        // the tail counted its own segments
+        uint64_t segs = launched > 0 ? n : 0;
+        const uint32_t counted = tail ? (launched > 0 ? launched - 1 : 0) : launched;
+        for (uint32_t k = 0; k < counted; ++k) segs += w.host_counts[k];
+        st.segments += segs + tail_segs;
     }
     if (timed) {
         AUR_HIP(hipEventRecord(t_end, c->stream));
@@ -485,12 +500,13 @@ int aurora_synchronize(aurora_ctx* c) {
 
 void* aurora_stream(aurora_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
-int aurora_set_launch(aurora_ctx* c, int ext_blocks_per_cu, int shade_blocks_per_cu) {
+int aurora_set_launch(aurora_ctx* c, int ext_blocks_per_cu, int shade_blocks_per_cu, uint32_t tail_paths) {
     if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
     if (ext_blocks_per_cu < 1 || ext_blocks_per_cu > 64 || shade_blocks_per_cu < 1 || shade_blocks_per_cu > 64)
         return fail(c, AURORA_ERR_INVALID, "blocks per CU must be in [1, 64]");
     c->ext_grid = c->cus * ext_blocks_per_cu;
     c->shade_grid = c->cus * shade_blocks_per_cu;
+    c->tail_paths = tail_paths;
     return AURORA_OK;
 }
 

@@ -16,7 +16,9 @@ constexpr int kPools = 8;
 constexpr uint32_t kPoolStride = 32;
 // render counters: [0], [1] queue counts, pools from kPoolBase
 constexpr uint32_t kPoolBase = 32;
-constexpr size_t kCounterBytes = (kPoolBase + kPools * kPoolStride) * 4;
+// 64-bit segment count of the fused tail kernel
+constexpr uint32_t kTailSegs = kPoolBase + kPools * kPoolStride;
+constexpr size_t kCounterBytes = (kTailSegs + 32) * 4;
 
 struct Status {
     int code = AURORA_OK;
@@ -120,6 +122,8 @@ struct Workspace {
 Status launch_raygen(const DevScene& sc, const RenderArgs& r, Workspace& ws, uint32_t n_paths, hipStream_t s);
 Status launch_shade(const DevScene& sc, const RenderArgs& r, Workspace& ws, int cur, uint32_t depth, int grid,
                     hipStream_t s);
+Status launch_tail(const DevScene& sc, const RenderArgs& r, Workspace& ws, int cur, uint32_t depth, int grid,
+                   hipStream_t s);
 Status launch_accumulate(const RenderArgs& r, Workspace& ws, float* rgb, float* alb, float* nrm, hipStream_t s);
 
 }  // namespace aur

@@ -14,6 +14,7 @@
 // (camera basis :339-343, pixel mapping :364-366) and the bounce loop of
 // rt.cpp:251-334; reflect/refract from src/basic/vec3.cpp:182-199.
 #include "internal.hpp"
+#include "traverse.hpp"
 
 namespace aur {
 
@@ -29,6 +30,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderArgs r, uint32_t n, flo
         counters[1] = 0u;  // queue 1 count
     }
     if (p < kPools) counters[kPoolBase + kPoolStride * p] = 0u;  // extend work pools
+    if (p == 0) *reinterpret_cast<unsigned long long*>(counters + kTailSegs) = 0ull;
     if (p >= n) return;
     const uint32_t WH = r.cam.W * r.cam.H;
     const uint32_t s_local = p / WH;
@@ -89,22 +91,16 @@ struct ShadeBufs {
 
 struct ShadeOut {
     v3 p, nd, beta;
-    uint32_t path;
     bool alive;
 };
 
-// one path: emission or background on termination, else scatter
-__device__ __forceinline__ ShadeOut shade_one(const DevScene& sc, const RenderArgs& r, const ShadeBufs& b,
-                                              uint32_t i, uint32_t depth, bool aov, uint32_t WH) {
+// One shading event of path `path` at segment `depth`: background or light
+// emission on termination (written once to L[path]), else scatter.
+__device__ __forceinline__ ShadeOut shade_core(const DevScene& sc, const RenderArgs& r, const ShadeBufs& b,
+                                               v3 org, v3 dir, v3 beta, float t, int prim, uint32_t path,
+                                               uint32_t depth, bool aov, uint32_t WH) {
     ShadeOut out;
     out.alive = false;
-    const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
-    const float2 H = b.hit[i];
-    const uint32_t path = __float_as_uint(O.w);
-    out.path = path;
-    const v3 org = xyz(O), dir = xyz(D);
-    v3 beta = xyz(B);
-    const int prim = __float_as_int(H.y);
     const uint32_t s_local = path / WH;
     const uint32_t pixel = path - s_local * WH;
     const uint32_t sample = r.sample_base + s_local;
@@ -119,7 +115,7 @@ __device__ __forceinline__ ShadeOut shade_one(const DevScene& sc, const RenderAr
         return out;
     }
     const uint32_t id = (uint32_t)prim;
-    const v3 p = org + H.x * dir;
+    const v3 p = org + t * dir;
     v3 outward;
     uint32_t mi;
     const bool is_sphere = id < sc.n_spheres;
@@ -193,12 +189,10 @@ __device__ __forceinline__ ShadeOut shade_one(const DevScene& sc, const RenderAr
     return out;
 }
 
-constexpr int kShadeItems = 4;
-constexpr uint32_t kShadeTile = kBlock * kShadeItems;
-
-// Grid-stride over tiles of 1024 live paths.  Survivors of a tile are
-// appended to the next queue with one atomic per block: counts are scanned
-// within each wave (shuffles) and across the 4 waves (LDS).
+// Grid-stride over the live queue, one path per thread.  Survivors are
+// appended to the next queue with one atomic per block and iteration: counts
+// are scanned within each wave (ballot popcounts) and across the 4 waves
+// (LDS).
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, RenderArgs r, ShadeBufs b, uint32_t depth) {
     __shared__ uint32_t s_wave[kBlock / 64];
     __shared__ uint32_t s_base;
@@ -206,25 +200,22 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, RenderArgs r, Sha
     uint32_t* next_count = &b.counters[b.cur ^ 1];
     if (blockIdx.x == 0 && threadIdx.x < kPools) b.counters[kPoolBase + kPoolStride * threadIdx.x] = 0u;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t WH = r.cam.W * r.cam.H;
     const bool aov = r.aov && depth == 0;
-    for (uint32_t tile = blockIdx.x * kShadeTile; tile < n; tile += gridDim.x * kShadeTile) {
-        ShadeOut o[kShadeItems];
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int k = 0; k < kShadeItems; ++k) {
-            const uint32_t i = tile + k * kBlock + tid;
-            if (i < n) o[k] = shade_one(sc, r, b, i, depth, aov, WH);
-            else o[k].alive = false;
-            cnt += o[k].alive ? 1u : 0u;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + tid;
+        ShadeOut o;
+        o.alive = false;
+        uint32_t path = 0;
+        if (i < n) {
+            const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
+            const float2 H = b.hit[i];
+            path = __float_as_uint(O.w);
+            o = shade_core(sc, r, b, xyz(O), xyz(D), xyz(B), H.x, __float_as_int(H.y), path, depth, aov, WH);
         }
-        uint32_t incl = cnt;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t v = __shfl_up(incl, off);
-            if (lane >= off) incl += v;
-        }
-        if (lane == 63) s_wave[wave] = incl;
+        const uint64_t m = __ballot(o.alive);
+        if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
         __syncthreads();
         uint32_t before = 0, total = 0;
 #pragma unroll
@@ -235,18 +226,48 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, RenderArgs r, Sha
         }
         if (tid == 0) s_base = total ? atomicAdd(next_count, total) : 0u;
         __syncthreads();
-        uint32_t slot = s_base + before + incl - cnt;
-#pragma unroll
-        for (int k = 0; k < kShadeItems; ++k) {
-            if (o[k].alive) {
-                b.no[slot] = make_float4(o[k].p.x, o[k].p.y, o[k].p.z, __uint_as_float(o[k].path));
-                b.nd[slot] = make_float4(o[k].nd.x, o[k].nd.y, o[k].nd.z, 0.0f);
-                b.nb[slot] = make_float4(o[k].beta.x, o[k].beta.y, o[k].beta.z, 0.0f);
-                ++slot;
-            }
+        if (o.alive) {
+            const uint32_t slot = s_base + before + (uint32_t)__popcll(m & lt);
+            b.no[slot] = make_float4(o.p.x, o.p.y, o.p.z, __uint_as_float(path));
+            b.nd[slot] = make_float4(o.nd.x, o.nd.y, o.nd.z, 0.0f);
+            b.nb[slot] = make_float4(o.beta.x, o.beta.y, o.beta.z, 0.0f);
         }
-        __syncthreads();  // s_wave / s_base reused by the next tile
+        __syncthreads();  // s_wave / s_base are reused by the next iteration
     }
+}
+
+// Fused tail: once the live queue is short, each thread carries one path
+// from segment `depth` to termination (traverse -> shade -> ...), so the
+// last bounces cost one launch instead of two per bounce.  Same arithmetic
+// as the extend/shade pair, so results are unchanged.
+__global__ __launch_bounds__(kBlock) void k_tail(DevScene sc, RenderArgs r, ShadeBufs b, uint32_t depth0,
+                                                 uint32_t* spill, uint32_t spill_stride) {
+    __shared__ uint32_t stk[kTravLds * kBlock];
+    const uint32_t n = b.counters[b.cur];
+    const uint32_t WH = r.cam.W * r.cam.H;
+    uint32_t* my_spill = spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
+    unsigned long long segs = 0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
+        const uint32_t path = __float_as_uint(O.w);
+        v3 org = xyz(O), dir = xyz(D), beta = xyz(B);
+        for (uint32_t depth = depth0; depth < r.max_depth; ++depth) {
+            ++segs;
+            float t;
+            int prim;
+            traverse_closest(sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill, spill_stride, t,
+                             prim);
+            const ShadeOut o = shade_core(sc, r, b, org, dir, beta, t, prim, path, depth, r.aov && depth == 0, WH);
+            if (!o.alive) break;
+            org = o.p;
+            dir = o.nd;
+            beta = o.beta;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) segs += __shfl_xor(segs, off);
+    if ((threadIdx.x & 63) == 0 && segs)
+        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kTailSegs), segs);
 }
 
 // per pixel: add the pass's samples in sample order to the caller's sums
@@ -303,6 +324,26 @@ Status launch_shade(const DevScene& sc, const RenderArgs& r, Workspace& ws, int 
     b.counters = ws.counters.as<uint32_t>();
     b.cur = cur;
     k_shade<<<grid, kBlock, 0, s>>>(sc, r, b, depth);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status launch_tail(const DevScene& sc, const RenderArgs& r, Workspace& ws, int cur, uint32_t depth, int grid,
+                   hipStream_t s) {
+    ShadeBufs b;
+    b.qo = ws.q_o[cur].as<float4>();
+    b.qd = ws.q_d[cur].as<float4>();
+    b.qb = ws.q_b[cur].as<float4>();
+    b.hit = nullptr;
+    b.no = b.nd = b.nb = nullptr;
+    b.L = ws.L.as<float4>();
+    b.aov_a = ws.aov_a.as<float4>();
+    b.aov_n = ws.aov_n.as<float4>();
+    b.counters = ws.counters.as<uint32_t>();
+    b.cur = cur;
+    const size_t need = (size_t)grid * kBlock * (kTravMaxStack - kTravLds) * 4;
+    if (need > ws.spill.bytes) return Status{AURORA_ERR_RUNTIME, "tail grid exceeds the stack spill buffer"};
+    k_tail<<<grid, kBlock, 0, s>>>(sc, r, b, depth, ws.spill.as<uint32_t>(), (uint32_t)(grid * kBlock));
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }

@@ -191,10 +191,13 @@ const char* aurora_last_error(const aurora_ctx* ctx);
 int aurora_synchronize(aurora_ctx* ctx);
 /* The context's HIP stream, as a void* (hipStream_t). */
 void* aurora_stream(aurora_ctx* ctx);
-/* Persistent grid sizes of the traversal and shade kernels, in blocks per
- * compute unit (default 8 each; both in [1, 64]).  Changes speed only. */
+/* Launch shape: persistent grid sizes of the traversal and shade kernels in
+ * blocks per compute unit (default 8 each; both in [1, 64]), and the live
+ * path count at or below which the remaining bounces of a pass run fused in
+ * one kernel (default 65536; 0 = never).  Changes speed only, never
+ * results. */
 int aurora_set_launch(aurora_ctx* ctx, int extend_blocks_per_cu,
-                      int shade_blocks_per_cu);
+                      int shade_blocks_per_cu, uint32_t tail_paths);
 
 /* Validate and upload a flattened scene (replaces are::ObjectSet,
  * include/object/object_set.h:10-12, with its Triangle, Material and Texture

@@ -117,13 +117,19 @@ def test_render_radiance_bit_exact(ctx, name, mk, mkcam, rs):
     arr = mk()
     cam = mkcam()
     g, gs, o, os_ = both(ctx, arr)
-    grgb, _, _, st = g.render(cam, rs)
     orgb, _, _, segs = o.render(cam, rs)
-    eq = (grgb.view(np.uint32) == orgb.view(np.uint32)).all(-1)
-    say(name, "pixels bit-equal:", eq.mean(), "segments gpu/cpu", st["segments"], segs)
-    assert st["paths"] == cam.image_width * cam.image_height * rs.spp
-    assert st["segments"] == segs
-    assert eq.all()
+    # pure wavefront, wavefront then fused tail, fused from the first bounce
+    for tail in (0, 2048, 1 << 30):
+        ctx.set_launch(tail_paths=tail)
+        try:
+            grgb, _, _, st = g.render(cam, rs)
+        finally:
+            ctx.set_launch()
+        eq = (grgb.view(np.uint32) == orgb.view(np.uint32)).all(-1)
+        say(name, "tail", tail, "pixels bit-equal:", eq.mean(), "segments gpu/cpu", st["segments"], segs)
+        assert st["paths"] == cam.image_width * cam.image_height * rs.spp
+        assert st["segments"] == segs
+        assert eq.all()
 
 
 def test_render_aov_and_progressive(ctx):
@@ -132,11 +138,16 @@ def test_render_aov_and_progressive(ctx):
     cam = scenes.cornell_camera(32)
     g, gs, o, os_ = both(ctx, arr)
     rs = RenderSettings(spp=6, aov=True, spp_per_pass=4)
-    grgb, galb, gnrm, _ = g.render(cam, rs)
     orgb, oalb, onrm, _ = o.render(cam, rs)
-    assert np.array_equal(grgb, orgb)
-    assert np.array_equal(galb, oalb)
-    assert np.array_equal(gnrm, onrm)
+    for tail in (0, 1 << 30):
+        ctx.set_launch(tail_paths=tail)
+        try:
+            grgb, galb, gnrm, _ = g.render(cam, rs)
+        finally:
+            ctx.set_launch()
+        assert np.array_equal(grgb, orgb)
+        assert np.array_equal(galb, oalb)
+        assert np.array_equal(gnrm, onrm)
     # two calls with sample_offset == one call (sums continue in sample order)
     a, _, _, _ = g.render(cam, RenderSettings(spp=2))
     a, _, _, _ = g.render(cam, RenderSettings(spp=4, sample_offset=2), rgb=a)
