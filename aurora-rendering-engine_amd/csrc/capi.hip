@@ -318,7 +318,12 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
         return Status::ok();
     }
     const int ext_grid = c->ext_grid;
-    AUR_TRY(ensure_workspace(s, S * WH, ext_grid));
+    // the fused tail runs one path per thread on up to 4x the resident grid
+    const int tail_grid = (int)std::min<uint64_t>(4ull * ext_grid,
+                                                  std::max<uint64_t>(1, ((uint64_t)std::min<uint64_t>(
+                                                                            c->tail_paths, (uint64_t)S * WH) +
+                                                                        255) / 256));
+    AUR_TRY(ensure_workspace(s, S * WH, std::max(ext_grid, tail_grid)));
     Workspace& w = s->ws;
     r.seed = rd->seed;
     r.max_depth = rd->max_depth;
@@ -368,16 +373,16 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
                 ub = w.host_counts[depth - 2];
                 if (ub == 0) break;
             }
+            // one item per thread while the queue is shorter than the chip
+            const uint64_t blocks = std::max<uint64_t>(1, ((uint64_t)ub + 255) / 256);
             if (ub <= c->tail_paths) {
-                const int tg = (int)std::min<uint64_t>((uint64_t)ext_grid, ((uint64_t)ub + 255) / 256);
-                AUR_TRY(launch_tail(s->view, r, w, cur, depth, std::max(1, tg), c->stream));
+                const int tg = (int)std::min<uint64_t>((uint64_t)tail_grid, blocks);
+                AUR_TRY(launch_tail(s->view, r, w, cur, depth, tg, c->stream));
                 tail = true;
                 break;
             }
-            const int eg = (int)std::min<uint64_t>((uint64_t)ext_grid,
-                                                   std::max<uint64_t>(1, ((uint64_t)ub + 1023) / 1024));
-            const int sg = (int)std::min<uint64_t>((uint64_t)c->shade_grid,
-                                                   std::max<uint64_t>(1, ((uint64_t)ub + 2047) / 2048));
+            const int eg = (int)std::min<uint64_t>((uint64_t)ext_grid, blocks);
+            const int sg = (int)std::min<uint64_t>((uint64_t)c->shade_grid, blocks);
             ExtendArgs a{};
             a.ray_o = w.q_o[cur].as<float4>();
             a.ray_d = w.q_d[cur].as<float4>();

@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--width", type=int, default=1024)
     ap.add_argument("--spp-per-step", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tail-paths", type=int, default=None, help="fused-tail switch (default: library's)")
+    ap.add_argument("--sweep-tail", action="store_true", help="time several fused-tail switch points")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -64,6 +66,8 @@ def main():
     from aurora_amd.core import RenderSettings
 
     ctx = aurora_amd.Context(0)
+    if args.tail_paths is not None:
+        ctx.set_launch(tail_paths=args.tail_paths)
     arr = scenes.cornell_box()
     scene = ctx.upload(arr)
     bvh = scene.build_bvh()
@@ -77,6 +81,19 @@ def main():
                             count_fetches=count)
         return scene.render_device(cam, rs, rgb)
 
+    if args.sweep_tail:
+        for tp in (0, 65536, 262144, 1 << 20, 4 << 20, 1 << 30):
+            ctx.set_launch(tail_paths=tp)
+            step(0)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for i in range(5):
+                step(1 + i)
+            ctx.synchronize()
+            dt = (time.perf_counter() - t0) / 5
+            print(f"sweep tail_paths={tp}: {dt * 1e3:.2f} ms/step, {W * H * S / dt / 1e6:.1f} Mpaths/s",
+                  file=sys.stderr, flush=True)
+        ctx.set_launch(tail_paths=args.tail_paths if args.tail_paths is not None else 65536)
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()
