@@ -86,6 +86,11 @@ class BvhStats(C.Structure):
     ]
 
 
+class TraceStats(C.Structure):
+    _fields_ = [("extend_ms", C.c_double), ("nodes_fetched", C.c_uint64), ("spheres_fetched", C.c_uint64),
+                ("quads_fetched", C.c_uint64)]
+
+
 class CameraDesc(C.Structure):
     _fields_ = [
         ("image_width", C.c_uint32),
@@ -151,6 +156,8 @@ EXPORTS = [
     ("aurora_trace_closest", C.c_int, [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp]),
     ("aurora_trace_closest_device", C.c_int, [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp, _vp]),
     ("aurora_trace_brute_device", C.c_int, [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp]),
+    ("aurora_trace_closest_device_stats", C.c_int,
+     [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp, C.c_int, C.POINTER(TraceStats)]),
     ("aurora_render", C.c_int,
      [_vp, C.POINTER(CameraDesc), C.POINTER(RenderDesc), _vp, _vp, _vp, C.POINTER(RenderStats)]),
     ("aurora_render_device", C.c_int,

@@ -162,7 +162,7 @@ class Context:
         except Exception:
             pass
 
-    def set_launch(self, extend_blocks_per_cu=8, shade_blocks_per_cu=8, tail_paths=65536):
+    def set_launch(self, extend_blocks_per_cu=8, shade_blocks_per_cu=8, tail_paths=1 << 20):
         self.check(self.lib.aurora_set_launch(self.h, extend_blocks_per_cu, shade_blocks_per_cu, tail_paths))
 
     def synchronize(self):
@@ -238,6 +238,15 @@ class Scene:
                                                       prim.data_ptr(), t.data_ptr(),
                                                       counts.data_ptr() if counts is not None else None)
         self.ctx.check(rc)
+
+    def trace_device_timed(self, o, d, prim, t, tmin=0.0, tmax=float("inf"), count_fetches=False):
+        """Device trace with the traversal kernel timed by HIP events;
+        returns the stats dict (extend_ms, fetch totals when counted)."""
+        st = abi.TraceStats()
+        self.ctx.check(self.lib.aurora_trace_closest_device_stats(
+            self.h, o.data_ptr(), d.data_ptr(), o.shape[0], tmin, tmax, prim.data_ptr(), t.data_ptr(),
+            1 if count_fetches else 0, C.byref(st)))
+        return stats_dict(st)
 
     def render(self, cam: Camera, rs: RenderSettings, rgb=None, albedo=None, normal=None):
         """Host-buffer render; returns (rgb_sum, albedo_sum, normal_sum, stats)."""

@@ -285,3 +285,17 @@ def random_rays(n, lo, hi, seed=0):
     g = np.stack([r1 * np.cos(2 * np.pi * u2), r1 * np.sin(2 * np.pi * u2), r2 * np.cos(2 * np.pi * u4)], axis=1)
     g /= np.linalg.norm(g, axis=1, keepdims=True)
     return o.astype(np.float32), g.astype(np.float32)
+
+
+def random_rays_torch(n, lo, hi, seed=0, device="cuda"):
+    """Device-side ray set for timing runs: origins uniform in [lo, hi]^3,
+    directions normalised Gaussian triples (torch generator seeded `seed`)."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    lo = torch.tensor(lo, dtype=torch.float32, device=device)
+    hi = torch.tensor(hi, dtype=torch.float32, device=device)
+    o = lo + (hi - lo) * torch.rand((n, 3), generator=g, device=device)
+    d = torch.randn((n, 3), generator=g, device=device)
+    d = d / d.norm(dim=1, keepdim=True).clamp_min(1e-12)
+    return o.contiguous(), d.contiguous()

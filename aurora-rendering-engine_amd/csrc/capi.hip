@@ -27,7 +27,7 @@ struct aurora_ctx {
     int cus = 0;
     int ext_grid = 0;    // resident blocks of the extend kernel
     int shade_grid = 0;
-    uint32_t tail_paths = 65536;  // fused-tail switch: live paths at or below
+    uint32_t tail_paths = 1u << 20;  // fused-tail switch: live paths at or below
 };
 
 struct aurora_scene {
@@ -232,7 +232,8 @@ Status ensure_spill(DevBuf& spill, int grid) {
 }
 
 Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n, float tmin, float tmax,
-                    int32_t* prim, float* t, uint32_t* counts, bool brute) {
+                    int32_t* prim, float* t, uint32_t* counts, bool brute, aurora_trace_stats* ts = nullptr,
+                    bool count_fetches = false) {
     aurora_ctx* c = s->ctx;
     if (!brute && !s->built) return Status{AURORA_ERR_STATE, "BVH not built: call aurora_bvh_build first"};
     if (n == 0) return Status::ok();
@@ -261,9 +262,40 @@ Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n,
         a.tmax = tmax;
         a.hit = s->tr_hit.as<float2>();
         a.counts = counts;
+        if (count_fetches) {
+            AUR_TRY(s->totals.alloc(64));
+            AUR_HIP(hipMemsetAsync(s->totals.p, 0, 24, c->stream));
+            a.totals = s->totals.as<unsigned long long>();
+        }
         a.spill = s->tr_spill.as<uint32_t>();
         a.spill_stride = (uint32_t)(grid * extend_block());
-        AUR_TRY(launch_extend(s->view, a, grid, c->stream));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (ts) {
+            AUR_HIP(hipEventCreate(&e0));
+            AUR_HIP(hipEventCreate(&e1));
+            AUR_HIP(hipEventRecord(e0, c->stream));
+        }
+        Status ls = launch_extend(s->view, a, grid, c->stream);
+        if (ts) {
+            Status es = hip_status(hipEventRecord(e1, c->stream), "hipEventRecord");
+            if (!es.bad()) es = hip_status(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.0f;
+            if (!es.bad()) es = hip_status(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            AUR_TRY(ls);
+            AUR_TRY(es);
+            *ts = aurora_trace_stats{};
+            ts->extend_ms = ms;
+            if (count_fetches) {
+                unsigned long long tot[3];
+                AUR_HIP(hipMemcpy(tot, s->totals.p, 24, hipMemcpyDeviceToHost));
+                ts->nodes_fetched = tot[0];
+                ts->spheres_fetched = tot[1];
+                ts->quads_fetched = tot[2];
+            }
+        }
+        AUR_TRY(ls);
     }
     k_unpack_hits<<<blocks, 256, 0, c->stream>>>(s->tr_hit.as<float2>(), prim, t, n);
     AUR_HIP(hipGetLastError());
@@ -646,6 +678,19 @@ int aurora_trace_closest_device(aurora_scene* s, const float* o, const float* d,
     if (n && (!o || !d || !prim || !t)) return fail(s->ctx, AURORA_ERR_INVALID, "NULL ray or output buffer");
     CK(s->ctx, hip_status(hipSetDevice(s->ctx->device), "hipSetDevice"));
     CK(s->ctx, trace_device(s, o, d, n, tmin, tmax, prim, t, counts, false));
+    return AURORA_OK;
+}
+
+int aurora_trace_closest_device_stats(aurora_scene* s, const float* o, const float* d, uint32_t n, float tmin,
+                                      float tmax, int32_t* prim, float* t, int count_fetches,
+                                      aurora_trace_stats* stats) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    if (n && (!o || !d || !prim || !t)) return fail(s->ctx, AURORA_ERR_INVALID, "NULL ray or output buffer");
+    aurora_trace_stats local{};
+    CK(s->ctx, hip_status(hipSetDevice(s->ctx->device), "hipSetDevice"));
+    CK(s->ctx, trace_device(s, o, d, n, tmin, tmax, prim, t, nullptr, false, &local, count_fetches != 0));
+    CK(s->ctx, hip_status(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize"));
+    if (stats) *stats = local;
     return AURORA_OK;
 }
 

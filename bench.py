@@ -43,6 +43,55 @@ def cpu_baseline(arr, cam, spp, seconds=12.0):
                       f"{spp} spp, depth 50 ({paths} paths, {elapsed:.1f} s, FP32 oracle)"}
 
 
+def stress_section(ctx, n_rays=1 << 22, render_spp=1):
+    """Traversal stress scene: 2M spheres (log-uniform radii in [1e-3, 1e-1])
+    and 2M triangles in the unit cube, LBVH of 4M leaves (256 MB of nodes).
+    Incoherent rays (origins in the cube, uniform directions) and one pass
+    of the 1920x1080 depth-16 render; algorithmic bytes = 32 per node + 16
+    per sphere + 48 per triangle record fetched."""
+    import torch
+
+    from aurora_amd import scenes
+    from aurora_amd.core import RenderSettings
+
+    t0 = time.perf_counter()
+    arr = scenes.stress_scene()
+    scene = ctx.upload(arr)
+    bvh = scene.build_bvh()
+    o, d = scenes.random_rays_torch(n_rays, [-0.05] * 3, [1.05] * 3, seed=0)
+    prim = torch.empty(n_rays, dtype=torch.int32, device="cuda")
+    t = torch.empty(n_rays, dtype=torch.float32, device="cuda")
+    scene.trace_device_timed(o, d, prim, t, tmin=1e-4)  # warm-up
+    runs = [scene.trace_device_timed(o, d, prim, t, tmin=1e-4) for _ in range(3)]
+    ms = sum(r["extend_ms"] for r in runs) / len(runs)
+    cnt = scene.trace_device_timed(o, d, prim, t, tmin=1e-4, count_fetches=True)
+    alg = 32 * cnt["nodes_fetched"] + 16 * cnt["spheres_fetched"] + 48 * cnt["quads_fetched"]
+    trav = {"rays": n_rays, "extend_ms": ms, "rays_per_s": n_rays / (ms * 1e-3),
+            "nodes_per_ray": cnt["nodes_fetched"] / n_rays,
+            "prims_per_ray": (cnt["spheres_fetched"] + cnt["quads_fetched"]) / n_rays,
+            "alg_bytes": alg, "achieved_GBs": alg / (ms * 1e-3) / 1e9,
+            "frac_of_hbm_peak": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "hit_fraction": float((prim >= 0).float().mean().item())}
+    cam = scenes.stress_camera(1920, 1080)
+    rgb = torch.zeros((1080, 1920, 3), dtype=torch.float32, device="cuda")
+    rs = RenderSettings(spp=render_spp, max_depth=16, seed=0, background=(0.7, 0.8, 1.0), tmin=1e-4)
+    scene.render_device(cam, rs, rgb)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    st = scene.render_device(cam, rs, rgb)
+    ctx.synchronize()
+    wall = time.perf_counter() - t1
+    rc = scene.render_device(cam, RenderSettings(spp=render_spp, max_depth=16, seed=0, background=(0.7, 0.8, 1.0),
+                                                 tmin=1e-4, count_fetches=True), rgb)
+    ralg = 32 * rc["nodes_fetched"] + 16 * rc["spheres_fetched"] + 48 * rc["quads_fetched"]
+    render = {"paths": st["paths"], "paths_per_s": st["paths"] / wall, "rays_per_s": st["segments"] / wall,
+              "extend_ms": st["extend_ms"], "extend_achieved_GBs": ralg / (st["extend_ms"] * 1e-3) / 1e9}
+    scene.close()
+    return {"workload": "stress_2M_spheres_2M_triangles", "bvh_nodes": bvh["n_nodes"], "bvh_depth": bvh["depth"],
+            "bvh_build_ms": bvh["build_ms"], "traversal": trav, "render_1920x1080_depth16": render,
+            "setup_s": time.perf_counter() - t0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +102,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tail-paths", type=int, default=None, help="fused-tail switch (default: library's)")
     ap.add_argument("--sweep-tail", action="store_true", help="time several fused-tail switch points")
+    ap.add_argument("--no-stress", action="store_true", help="skip the traversal stress scene")
+    ap.add_argument("--only", choices=["stress", "copy"], default=None,
+                    help="profiling modes: the stress section alone, or a plain 1 GiB device copy "
+                         "(calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -65,7 +118,18 @@ def main():
     from aurora_amd import scenes
     from aurora_amd.core import RenderSettings
 
+    if args.only == "copy":
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": src.numel() * 4, "copies": 3}), flush=True)
+        return
     ctx = aurora_amd.Context(0)
+    if args.only == "stress":
+        print(json.dumps(stress_section(ctx)), flush=True)
+        return
     if args.tail_paths is not None:
         ctx.set_launch(tail_paths=args.tail_paths)
     arr = scenes.cornell_box()
@@ -93,7 +157,7 @@ def main():
             dt = (time.perf_counter() - t0) / 5
             print(f"sweep tail_paths={tp}: {dt * 1e3:.2f} ms/step, {W * H * S / dt / 1e6:.1f} Mpaths/s",
                   file=sys.stderr, flush=True)
-        ctx.set_launch(tail_paths=args.tail_paths if args.tail_paths is not None else 65536)
+        ctx.set_launch(tail_paths=args.tail_paths if args.tail_paths is not None else 1 << 20)
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()
@@ -141,6 +205,9 @@ def main():
                      "kernel": "k_extend", "alg_bytes_per_launch": alg_bytes / max(1, first["extend_launches"]),
                      "avg_launch_ms": first["extend_ms"] / max(1, first["extend_launches"])},
     }
+    if not args.no_stress:
+        scene.close()
+        out["stress"] = stress_section(ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(arr, cam, S)
     print(json.dumps(out), flush=True)

@@ -194,7 +194,7 @@ void* aurora_stream(aurora_ctx* ctx);
 /* Launch shape: persistent grid sizes of the traversal and shade kernels in
  * blocks per compute unit (default 8 each; both in [1, 64]), and the live
  * path count at or below which the remaining bounces of a pass run fused in
- * one kernel (default 65536; 0 = never).  Changes speed only, never
+ * one kernel (default 1048576; 0 = never).  Changes speed only, never
  * results. */
 int aurora_set_launch(aurora_ctx* ctx, int extend_blocks_per_cu,
                       int shade_blocks_per_cu, uint32_t tail_paths);
@@ -234,6 +234,22 @@ int aurora_trace_closest_device(aurora_scene* scene, const float* d_origins,
                                 const float* d_dirs, uint32_t n, float tmin,
                                 float tmax, int32_t* d_prim, float* d_t,
                                 uint32_t* d_counts);
+/* Device time and fetch totals of one traversal launch. */
+typedef struct aurora_trace_stats {
+    double extend_ms;          /* HIP-event time of the traversal kernel     */
+    uint64_t nodes_fetched;    /* 32-byte BVH nodes (count_fetches only)     */
+    uint64_t spheres_fetched;  /* 16-byte sphere records                     */
+    uint64_t quads_fetched;    /* 48-byte quad records                       */
+} aurora_trace_stats;
+/* aurora_trace_closest_device with the traversal launch timed on the
+ * context stream (synchronous on return); with count_fetches != 0 the
+ * counting kernel variant also sums the records fetched. */
+int aurora_trace_closest_device_stats(aurora_scene* scene,
+                                      const float* d_origins,
+                                      const float* d_dirs, uint32_t n,
+                                      float tmin, float tmax, int32_t* d_prim,
+                                      float* d_t, int count_fetches,
+                                      aurora_trace_stats* stats);
 /* Brute-force closest hit (every primitive tested, no BVH): the semantics
  * of experiments/rt.cpp:209-218.  Parity aid. Device buffers. */
 int aurora_trace_brute_device(aurora_scene* scene, const float* d_origins,
