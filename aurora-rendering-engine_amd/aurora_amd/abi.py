@@ -163,6 +163,17 @@ EXPORTS = [
     ("aurora_render_device", C.c_int,
      [_vp, C.POINTER(CameraDesc), C.POINTER(RenderDesc), _vp, _vp, _vp, C.POINTER(RenderStats)]),
     ("aurora_resolve_rgb8", C.c_int, [_vp, _u32, _u32, _u32, _vp]),
+    ("aurora_denoiser_create", C.c_int, [_vp, _u32, C.POINTER(_vp)]),
+    ("aurora_denoiser_destroy", None, [_vp]),
+    ("aurora_denoiser_param_count", C.c_uint64, [_vp]),
+    ("aurora_denoiser_get_params", C.c_int, [_vp, _vp, C.c_uint64]),
+    ("aurora_denoiser_set_params", C.c_int, [_vp, _vp, C.c_uint64]),
+    ("aurora_denoise", C.c_int, [_vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp]),
+    ("aurora_denoiser_loss_grad", C.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _u32, C.POINTER(C.c_float), _vp]),
+    ("aurora_denoiser_train_step", C.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _f, C.POINTER(C.c_float)]),
+    ("aurora_selftest_mfma", C.c_int, [_vp, _vp, _vp, _vp]),
 ]
 
 _lib = None

@@ -489,6 +489,11 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
 
 }  // namespace
 
+// accessors for the other translation units of the C ABI (denoise.hip)
+hipStream_t ctx_stream(aurora_ctx* c) { return c->stream; }
+int ctx_device(aurora_ctx* c) { return c->device; }
+int ctx_fail(aurora_ctx* c, int code, const std::string& msg) { return fail(c, code, msg); }
+
 extern "C" {
 
 int aurora_abi_version(void) { return AURORA_ABI_VERSION; }

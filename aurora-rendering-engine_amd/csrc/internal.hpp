@@ -37,8 +37,8 @@ struct Status {
 
 #define AUR_TRY(expr)             \
     do {                          \
-        Status s_ = (expr);       \
-        if (s_.bad()) return s_;  \
+        Status aur_st_ = (expr);  \
+        if (aur_st_.bad()) return aur_st_; \
     } while (0)
 
 // Device buffer owned by the library.

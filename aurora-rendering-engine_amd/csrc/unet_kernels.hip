@@ -1,0 +1,520 @@
+// unet_kernels.hip — denoiser kernels for gfx950: 3x3 convolution as an
+// implicit GEMM on MFMA (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32
+// accumulation) for the forward pass, the data gradient and the weight
+// gradient, plus the elementwise pieces of the U-Net (average pooling,
+// concat/upsample gradient split, ReLU masks, input preparation, loss, Adam).
+//
+// Tensors are NHWC.  Activations and the gradients fed to MFMA are BF16;
+// gradients that are accumulated are FP32.  A convolution input is the
+// concatenation of a full-resolution tensor A (Ca channels) and a
+// half-resolution tensor B (Cb channels) read through nearest upsampling, so
+// the decoder's upsample + skip concat is never materialised.  Ca, Cb and
+// Cout are multiples of 16.  Weights of a layer are [Cout][9][Cin] (k =
+// tap*Cin + c, tap = 3*(dy+1) + (dx+1)), padded to Kpad = ceil(9*Cin/32)*32.
+//
+// MFMA operand layout (16x16x32, wave64): lane l holds A[l%16][8*(l/16)+j]
+// and B[8*(l/16)+j][l%16] for j = 0..7 and returns D[4*(l/16)+r][l%16] for
+// r = 0..3 (checked on the device by aurora_selftest_mfma).
+//
+// No reference counterpart: the reference has no learned component.
+#include "unet.hpp"
+
+#include <algorithm>
+
+namespace aur {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16;
+
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ u16 f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)((u >> 16) | 0x40);  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                                      // round to nearest even
+    return (u16)(u >> 16);
+}
+
+union Pack8 {
+    uint4 u;
+    bf16x8 v;
+    u16 h[8];
+};
+
+__device__ __forceinline__ bf16x8 zero8() {
+    Pack8 p;
+    p.u = make_uint4(0, 0, 0, 0);
+    return p.v;
+}
+
+// 8 consecutive channels [c, c+8) of the concatenated input at (n, y, x);
+// zero outside the image.
+__device__ __forceinline__ bf16x8 gather8(const ConvIn& in, int n, int y, int x, int H, int W, int c) {
+    if (y < 0 || y >= H || x < 0 || x >= W) return zero8();
+    Pack8 p;
+    if (c < in.ca) {
+        const u16* src = in.a + (((size_t)n * H + y) * W + x) * in.ca + c;
+        p.u = *reinterpret_cast<const uint4*>(src);
+    } else {
+        const int h2 = H >> 1, w2 = W >> 1;
+        const u16* src = in.b + (((size_t)n * h2 + (y >> 1)) * w2 + (x >> 1)) * in.cb + (c - in.ca);
+        p.u = *reinterpret_cast<const uint4*>(src);
+    }
+    return p.v;
+}
+
+// one channel of the concatenated input (weight-gradient operand gather)
+__device__ __forceinline__ u16 gather1(const ConvIn& in, int n, int y, int x, int H, int W, int c) {
+    if (y < 0 || y >= H || x < 0 || x >= W) return 0;
+    if (c < in.ca) return in.a[(((size_t)n * H + y) * W + x) * in.ca + c];
+    const int h2 = H >> 1, w2 = W >> 1;
+    return in.b[(((size_t)n * h2 + (y >> 1)) * w2 + (x >> 1)) * in.cb + (c - in.ca)];
+}
+
+// ------------------------------------------------------------ forward ---
+// Block: 4 waves, 64 consecutive pixels of one image row; each wave a
+// 16-pixel x Cout tile (Cout/16 accumulators).  K = 9*Cin in chunks of 32.
+template <int NT, bool RELU, bool OUT_F32, bool ACCUM>
+__global__ __launch_bounds__(256) void k_conv3x3(ConvIn in, const u16* __restrict__ wk, int kpad,
+                                                 const float* __restrict__ bias, void* out, int N, int H, int W,
+                                                 int out_stride, int co_base) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = blockIdx.x / ((W + 63) / 64);
+    const int xb = (blockIdx.x % ((W + 63) / 64)) * 64 + wave * 16;
+    const int n = row / H, y = row % H;
+    if (n >= N || xb >= W) return;
+    const int cin = in.ca + in.cb;
+    const int K = 9 * cin;
+    const int m = lane & 15;      // pixel of the A fragment
+    const int kg = lane >> 4;     // k group of 8
+    const int px = xb + m;
+    floatx4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < kpad; k0 += 32) {
+        const int k = k0 + 8 * kg;
+        bf16x8 a;
+        if (k < K) {
+            const int tap = k / cin, c = k - tap * cin;
+            a = gather8(in, n, y + tap / 3 - 1, px + tap % 3 - 1, H, W, c);
+        } else {
+            a = zero8();
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            Pack8 b;
+            b.u = *reinterpret_cast<const uint4*>(wk + (size_t)(t * 16 + m) * kpad + k);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b.v, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + m;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int x = xb + 4 * kg + r;
+            if (x >= W) continue;
+            float v = acc[t][r] + bv;
+            if (RELU) v = fmaxf(v, 0.0f);
+            const size_t o = (((size_t)n * H + y) * W + x) * out_stride + co_base + co;
+            if (OUT_F32) {
+                float* of = static_cast<float*>(out);
+                of[o] = ACCUM ? of[o] + v : v;
+            } else {
+                static_cast<u16*>(out)[o] = f2bf(v);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------- weight gradient ---
+// dW[co][tap][ci] = sum_p g[p][co] * X[p + tap][ci] over the rows of this
+// block: MFMA with M = 16 output channels, N = 16 input channels, K = 32
+// pixels of a row.  Each of the 4 waves walks every 4th row; the partial
+// tiles are summed in LDS and written per block (reduced afterwards in a
+// fixed order, so the result is deterministic).
+__global__ __launch_bounds__(256) void k_conv3x3_dw(ConvIn in, const u16* __restrict__ g, int cout, int N, int H,
+                                                    int W, int rows_per_block, float* __restrict__ partial) {
+    __shared__ float red[4][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cin = in.ca + in.cb;
+    const int n_ci = cin / 16, n_co = cout / 16;
+    const int tile = blockIdx.y;  // (tap, co tile, ci tile)
+    const int tap = tile / (n_co * n_ci);
+    const int rem = tile % (n_co * n_ci);
+    const int co0 = (rem / n_ci) * 16, ci0 = (rem % n_ci) * 16;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int m = lane & 15, kg = lane >> 4;
+    floatx4 acc = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int r0 = blockIdx.x * rows_per_block;
+    const int r1 = min(N * H, r0 + rows_per_block);
+    for (int row = r0 + wave; row < r1; row += 4) {
+        const int n = row / H, y = row % H;
+        for (int x0 = 0; x0 < W; x0 += 32) {
+            Pack8 a, b;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int x = x0 + 8 * kg + j;
+                a.h[j] = x < W ? g[(((size_t)n * H + y) * W + x) * cout + co0 + m] : (u16)0;
+                b.h[j] = x < W ? gather1(in, n, y + dy, x + dx, H, W, ci0 + m) : (u16)0;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc, 0, 0, 0);
+        }
+    }
+    // D[4*kg + r][m]: row = co offset, col = ci offset
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[wave][(4 * kg + r) * 16 + m] = acc[r];
+    __syncthreads();
+    const int t = threadIdx.x;
+    const float s = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+    partial[((size_t)blockIdx.x * gridDim.y + tile) * 256 + t] = s;
+}
+
+// dW = sum over row blocks in order; scatter tiles into [Cout][9][Cin]
+__global__ void k_dw_reduce(const float* __restrict__ partial, int nblk, int ntiles, int cout, int cin,
+                            float* __restrict__ dw) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= ntiles * 256) return;
+    const int tile = idx / 256, e = idx % 256;
+    float s = 0.0f;
+    for (int b = 0; b < nblk; ++b) s += partial[((size_t)b * ntiles + tile) * 256 + e];
+    const int n_ci = cin / 16, n_co = cout / 16;
+    const int tap = tile / (n_co * n_ci), rem = tile % (n_co * n_ci);
+    const int co = (rem / n_ci) * 16 + e / 16, ci = (rem % n_ci) * 16 + e % 16;
+    dw[((size_t)co * 9 + tap) * cin + ci] = s;
+}
+
+// db[co] = sum_p g[p][co] (per-block partials, then an ordered sum)
+__global__ void k_db_partial(const u16* __restrict__ g, int cout, size_t npix, size_t per_block,
+                             float* __restrict__ partial) {
+    __shared__ float red[256];
+    const int co = threadIdx.x % cout, lane_group = threadIdx.x / cout, groups = 256 / cout;
+    float s = 0.0f;
+    const size_t p0 = blockIdx.x * per_block, p1 = min(npix, p0 + per_block);
+    if (lane_group < groups)
+        for (size_t p = p0 + lane_group; p < p1; p += groups) s += bf2f(g[p * cout + co]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < cout) {
+        float t = 0.0f;
+        for (int k = 0; k < groups; ++k) t += red[k * cout + threadIdx.x];
+        partial[(size_t)blockIdx.x * cout + threadIdx.x] = t;
+    }
+}
+__global__ void k_db_reduce(const float* __restrict__ partial, int nblk, int cout, float* __restrict__ db) {
+    const int co = threadIdx.x;
+    if (co >= cout) return;
+    float s = 0.0f;
+    for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * cout + co];
+    db[co] = s;
+}
+
+// ------------------------------------------------------- elementwise ---
+// g = bf16(grad * (act > 0))  (act null: no mask)
+__global__ void k_mask_bf16(const float* __restrict__ grad, const u16* __restrict__ act, size_t n,
+                            u16* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    float v = grad[i];
+    if (act && !(bf2f(act[i]) > 0.0f)) v = 0.0f;
+    out[i] = f2bf(v);
+}
+
+__global__ void k_avgpool(const u16* __restrict__ in, int N, int H, int W, int C, u16* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int h2 = H / 2, w2 = W / 2;
+    const size_t total = (size_t)N * h2 * w2 * C;
+    if (i >= total) return;
+    const int c = i % C;
+    size_t q = i / C;
+    const int x = q % w2;
+    q /= w2;
+    const int y = q % h2;
+    const int n = q / h2;
+    auto at = [&](int yy, int xx) { return bf2f(in[(((size_t)n * H + yy) * W + xx) * C + c]); };
+    const float s = (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + (at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
+    out[i] = f2bf(0.25f * s);
+}
+
+// gin[p] += 0.25 * gout[parent(p)]
+__global__ void k_avgpool_bwd(const float* __restrict__ gout, int N, int H, int W, int C, float* __restrict__ gin) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t total = (size_t)N * H * W * C;
+    if (i >= total) return;
+    const int c = i % C;
+    size_t q = i / C;
+    const int x = q % W;
+    q /= W;
+    const int y = q % H;
+    const int n = q / H;
+    gin[i] += 0.25f * gout[(((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+}
+
+// split the gradient of a concatenated input [A | up(B)]: A part added to
+// ga, B part summed over each 2x2 block into gb (written, or added)
+__global__ void k_split_grad(const float* __restrict__ d, int N, int H, int W, int ca, int cb, float* __restrict__ ga,
+                             float* __restrict__ gb, int gb_accum) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t na = (size_t)N * H * W * ca;
+    const size_t nb = (size_t)N * (H / 2) * (W / 2) * cb;
+    const int C = ca + cb;
+    if (i < na) {
+        const size_t p = i / ca;
+        ga[i] += d[p * C + i % ca];
+    } else if (i < na + nb) {
+        const size_t j = i - na;
+        const int c = j % cb;
+        size_t q = j / cb;
+        const int x = q % (W / 2);
+        q /= (W / 2);
+        const int y = q % (H / 2);
+        const int n = q / (H / 2);
+        auto at = [&](int yy, int xx) { return d[(((size_t)n * H + yy) * W + xx) * C + ca + c]; };
+        const float s = (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + (at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
+        gb[j] = gb_accum ? gb[j] + s : s;
+    }
+}
+
+// network input: 16 channels = log(1 + radiance mean), albedo mean, normal
+// mean, zeros; also keeps log(1 + radiance) for the residual output
+__global__ void k_prep(const float* __restrict__ rgb, const float* __restrict__ alb, const float* __restrict__ nrm,
+                       size_t npix, float inv_spp, u16* __restrict__ x, float* __restrict__ noisy_log) {
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= npix) return;
+    Pack8 lo, hi;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float l = log1pf(fmaxf(rgb[3 * p + c] * inv_spp, 0.0f));
+        noisy_log[3 * p + c] = l;
+        lo.h[c] = f2bf(l);
+        lo.h[3 + c] = f2bf(alb ? alb[3 * p + c] * inv_spp : 0.0f);
+    }
+    lo.h[6] = f2bf(nrm ? nrm[3 * p] * inv_spp : 0.0f);
+    lo.h[7] = f2bf(nrm ? nrm[3 * p + 1] * inv_spp : 0.0f);
+    hi.u = make_uint4(0, 0, 0, 0);
+    hi.h[0] = f2bf(nrm ? nrm[3 * p + 2] * inv_spp : 0.0f);
+    reinterpret_cast<uint4*>(x + 16 * p)[0] = lo.u;
+    reinterpret_cast<uint4*>(x + 16 * p)[1] = hi.u;
+}
+
+// denoised linear radiance = expm1(log(1 + noisy) + out[0..2])
+__global__ void k_output(const float* __restrict__ out16, const float* __restrict__ noisy_log, size_t npix,
+                         float* __restrict__ img) {
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= npix) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) img[3 * p + c] = fmaxf(expm1f(noisy_log[3 * p + c] + out16[16 * p + c]), 0.0f);
+}
+
+// loss = mean over pixels and channels of (pred - log(1 + target))^2 in
+// log space; g16 = dloss/dout (bf16), per-block partial loss
+__global__ void k_loss(const float* __restrict__ out16, const float* __restrict__ noisy_log,
+                       const float* __restrict__ target, size_t npix, float* __restrict__ g16f,
+                       float* __restrict__ loss_partial) {
+    __shared__ float red[256];
+    const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+    float l = 0.0f;
+    if (p < npix) {
+        const float scale = 2.0f / (float)(3 * npix);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            float gv = 0.0f;
+            if (c < 3) {
+                const float d = (noisy_log[3 * p + c] + out16[16 * p + c]) - log1pf(fmaxf(target[3 * p + c], 0.0f));
+                l += d * d;
+                gv = scale * d;
+            }
+            g16f[16 * p + c] = gv;
+        }
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_partial[blockIdx.x] = red[0];
+}
+
+// Adam on the FP32 master parameters
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       size_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// BF16 operand copies of one layer: forward [Cout][Kpad] (k = tap*Cin+ci)
+// and data-gradient [Cin][Kpad'] (k' = tap'*Cout+co, tap' = 8 - tap)
+__global__ void k_pack_weights(const float* __restrict__ w, int cout, int cin, int kpad, int kpad_t,
+                               u16* __restrict__ wk, u16* __restrict__ wt) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int nf = cout * kpad, nt = cin * kpad_t;
+    if (i < nf) {
+        const int co = i / kpad, k = i % kpad;
+        wk[i] = k < 9 * cin ? f2bf(w[(size_t)co * 9 * cin + k]) : (u16)0;
+    } else if (i < nf + nt) {
+        const int j = i - nf;
+        const int ci = j / kpad_t, k = j % kpad_t;
+        u16 v = 0;
+        if (k < 9 * cout) {
+            const int tap = k / cout, co = k % cout;
+            v = f2bf(w[((size_t)co * 9 + (8 - tap)) * cin + ci]);
+        }
+        wt[j] = v;
+    }
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ in, size_t n, u16* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = f2bf(in[i]);
+}
+
+// 16x16x32 MFMA on one tile: the lane layout self-test
+__global__ void k_mfma_selftest(const u16* A, const u16* B, float* D) {
+    const int l = threadIdx.x;
+    Pack8 a, b;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        a.h[j] = A[(l % 16) * 32 + 8 * (l / 16) + j];  // A is 16 x 32 row-major
+        b.h[j] = B[(8 * (l / 16) + j) * 16 + l % 16];  // B is 32 x 16 row-major
+    }
+    floatx4 acc = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) D[(4 * (l / 16) + r) * 16 + l % 16] = acc[r];
+}
+
+inline unsigned nblocks(size_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- host ---
+Status conv3x3_forward(const ConvIn& in, const uint16_t* wk_all, int kpad, const float* bias_all, void* out,
+                       bool out_f32, bool accum, bool relu, int N, int H, int W, int cout, hipStream_t s) {
+    if ((in.ca % 16) || (in.cb % 16) || (cout % 16) || cout < 16 || (in.ca + in.cb) < 16)
+        return Status{AURORA_ERR_INVALID, "conv3x3: channel counts must be multiples of 16"};
+    const unsigned grid = (unsigned)(N * H * ((W + 63) / 64));
+    // output channels in groups of at most 64 (4 MFMA tiles per wave)
+    for (int co_base = 0; co_base < cout; co_base += 64) {
+    const int nt = std::min(64, cout - co_base) / 16;
+    const uint16_t* wk = wk_all + (size_t)co_base * kpad;
+    const float* bias = bias_all ? bias_all + co_base : nullptr;
+#define AUR_CONV(NTv, R, F, A) \
+    k_conv3x3<NTv, R, F, A><<<grid, 256, 0, s>>>(in, wk, kpad, bias, out, N, H, W, cout, co_base)
+#define AUR_CONV_NT(R, F, A)          \
+    switch (nt) {                     \
+        case 1: AUR_CONV(1, R, F, A); break; \
+        case 2: AUR_CONV(2, R, F, A); break; \
+        case 3: AUR_CONV(3, R, F, A); break; \
+        default: AUR_CONV(4, R, F, A); break; \
+    }
+    if (!out_f32) {
+        if (relu) { AUR_CONV_NT(true, false, false) } else { AUR_CONV_NT(false, false, false) }
+    } else if (accum) {
+        AUR_CONV_NT(false, true, true)
+    } else if (relu) {
+        AUR_CONV_NT(true, true, false)
+    } else {
+        AUR_CONV_NT(false, true, false)
+    }
+#undef AUR_CONV_NT
+#undef AUR_CONV
+    }
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status conv3x3_weight_grad(const ConvIn& in, const uint16_t* g, int cout, int N, int H, int W, float* dw, float* db,
+                           DevBuf& scratch, hipStream_t s) {
+    const int cin = in.ca + in.cb;
+    const int ntiles = 9 * (cout / 16) * (cin / 16);
+    const int rows = N * H;
+    const int rows_per_block = 16;
+    const int nblk = (rows + rows_per_block - 1) / rows_per_block;
+    const size_t npix = (size_t)N * H * W;
+    const size_t per_block = 4096;
+    const int nbb = (int)((npix + per_block - 1) / per_block);
+    AUR_TRY(scratch.alloc(((size_t)nblk * ntiles * 256 + (size_t)nbb * cout) * 4));
+    float* part = scratch.as<float>();
+    float* dbpart = part + (size_t)nblk * ntiles * 256;
+    k_conv3x3_dw<<<dim3(nblk, ntiles), 256, 0, s>>>(in, g, cout, N, H, W, rows_per_block, part);
+    k_dw_reduce<<<nblocks((size_t)ntiles * 256), 256, 0, s>>>(part, nblk, ntiles, cout, cin, dw);
+    k_db_partial<<<nbb, 256, 0, s>>>(g, cout, npix, per_block, dbpart);
+    k_db_reduce<<<1, 64, 0, s>>>(dbpart, nbb, cout, db);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status mask_to_bf16(const float* grad, const uint16_t* act, size_t n, uint16_t* out, hipStream_t s) {
+    k_mask_bf16<<<nblocks(n), 256, 0, s>>>(grad, act, n, out);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+Status avgpool2(const uint16_t* in, int N, int H, int W, int C, uint16_t* out, hipStream_t s) {
+    k_avgpool<<<nblocks((size_t)N * (H / 2) * (W / 2) * C), 256, 0, s>>>(in, N, H, W, C, out);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+Status avgpool2_backward(const float* gout, int N, int H, int W, int C, float* gin, hipStream_t s) {
+    k_avgpool_bwd<<<nblocks((size_t)N * H * W * C), 256, 0, s>>>(gout, N, H, W, C, gin);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+Status split_concat_grad(const float* d, int N, int H, int W, int ca, int cb, float* ga, float* gb, bool gb_accum,
+                         hipStream_t s) {
+    const size_t total = (size_t)N * H * W * ca + (size_t)N * (H / 2) * (W / 2) * cb;
+    k_split_grad<<<nblocks(total), 256, 0, s>>>(d, N, H, W, ca, cb, ga, gb, gb_accum ? 1 : 0);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+Status prep_input(const float* rgb, const float* alb, const float* nrm, size_t npix, float inv_spp, uint16_t* x,
+                  float* noisy_log, hipStream_t s) {
+    k_prep<<<nblocks(npix), 256, 0, s>>>(rgb, alb, nrm, npix, inv_spp, x, noisy_log);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+Status output_image(const float* out16, const float* noisy_log, size_t npix, float* img, hipStream_t s) {
+    k_output<<<nblocks(npix), 256, 0, s>>>(out16, noisy_log, npix, img);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+Status loss_grad(const float* out16, const float* noisy_log, const float* target, size_t npix, float* g16f,
+                 float* loss_partial, hipStream_t s) {
+    k_loss<<<nblocks(npix), 256, 0, s>>>(out16, noisy_log, target, npix, g16f, loss_partial);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+Status adam_step(float* w, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2, float eps,
+                 int t, hipStream_t s) {
+    const float c1 = 1.0f / (1.0f - powf(b1, (float)t)), c2 = 1.0f / (1.0f - powf(b2, (float)t));
+    k_adam<<<nblocks(n), 256, 0, s>>>(w, g, m, v, n, lr, b1, b2, eps, c1, c2);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+Status pack_weights(const float* w, int cout, int cin, int kpad, int kpad_t, uint16_t* wk, uint16_t* wt,
+                    hipStream_t s) {
+    k_pack_weights<<<nblocks((size_t)cout * kpad + (size_t)cin * kpad_t), 256, 0, s>>>(w, cout, cin, kpad, kpad_t,
+                                                                                       wk, wt);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+Status f32_to_bf16(const float* in, size_t n, uint16_t* out, hipStream_t s) {
+    k_f32_to_bf16<<<nblocks(n), 256, 0, s>>>(in, n, out);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+Status mfma_selftest(const uint16_t* A, const uint16_t* B, float* D, hipStream_t s) {
+    k_mfma_selftest<<<1, 64, 0, s>>>(A, B, D);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+}  // namespace aur

@@ -279,6 +279,49 @@ int aurora_render_device(aurora_scene* scene, const aurora_camera_desc* cam,
 int aurora_resolve_rgb8(const float* rgb_sum, uint32_t width,
                         uint32_t height, uint32_t spp, uint8_t* out);
 
+/* ---- learned denoiser ---------------------------------------------------
+ * A 4-level U-Net (32/48/64/64 channels, 15 3x3 convolutions) on BF16 MFMA
+ * with FP32 accumulation, fed the noisy radiance mean and the albedo and
+ * normal AOVs of aurora_render (AURORA_RENDER_AOV) and predicting a residual
+ * in log(1 + radiance).  No reference counterpart (ARE has no learned
+ * component).  Parameters are a flat FP32 vector: per layer the weights
+ * [Cout][3x3 tap][Cin] then the biases. */
+typedef struct aurora_denoiser aurora_denoiser;
+/* He-normal initialisation from Philox(seed); output layer zero. */
+int aurora_denoiser_create(aurora_ctx* ctx, uint32_t seed,
+                           aurora_denoiser** out);
+void aurora_denoiser_destroy(aurora_denoiser* dn);
+uint64_t aurora_denoiser_param_count(const aurora_denoiser* dn);
+int aurora_denoiser_get_params(aurora_denoiser* dn, float* host, uint64_t n);
+int aurora_denoiser_set_params(aurora_denoiser* dn, const float* host,
+                               uint64_t n);
+/* Denoise one image: inputs are device sums over `spp` samples (W*H*3 each,
+ * as aurora_render_device writes them); out receives linear RGB means.
+ * width and height must be multiples of 8. */
+int aurora_denoise(aurora_denoiser* dn, const float* d_rgb_sum,
+                   const float* d_albedo_sum, const float* d_normal_sum,
+                   uint32_t spp, uint32_t width, uint32_t height,
+                   float* d_out);
+/* Loss (mean squared error in log(1 + radiance) against d_target, linear
+ * RGB means, batch x H x W x 3) and its gradient; host_grads (optional)
+ * receives the flat FP32 gradient. */
+int aurora_denoiser_loss_grad(aurora_denoiser* dn, const float* d_rgb_sum,
+                              const float* d_albedo_sum,
+                              const float* d_normal_sum, const float* d_target,
+                              uint32_t batch, uint32_t spp, uint32_t width,
+                              uint32_t height, float* loss, float* host_grads);
+/* loss_grad followed by one Adam step (beta 0.9 / 0.999, eps 1e-8). */
+int aurora_denoiser_train_step(aurora_denoiser* dn, const float* d_rgb_sum,
+                               const float* d_albedo_sum,
+                               const float* d_normal_sum,
+                               const float* d_target, uint32_t batch,
+                               uint32_t spp, uint32_t width, uint32_t height,
+                               float lr, float* loss);
+/* Self-test of the MFMA lane layout: D (16x16 FP32) = A (16x32 BF16 bits,
+ * row-major) x B (32x16). Host buffers. */
+int aurora_selftest_mfma(aurora_ctx* ctx, const uint16_t* A, const uint16_t* B,
+                         float* D);
+
 #ifdef __cplusplus
 }
 #endif
