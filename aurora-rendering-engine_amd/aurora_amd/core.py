@@ -9,6 +9,7 @@ reference's constructors throw (src/object/triangle.cpp:13-36,
 src/texture.cpp:11-43).
 """
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -145,6 +146,11 @@ class Context:
         if rc != abi.AURORA_OK:
             raise RuntimeError(f"aurora_create failed ({rc}): {self.lib.aurora_last_error(None).decode()}")
         self.h = h
+        self._children = weakref.WeakSet()  # scenes / denoisers: closed before the context
+
+    def adopt(self, child):
+        self._children.add(child)
+        return child
 
     def check(self, rc):
         if rc != abi.AURORA_OK:
@@ -153,6 +159,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for ch in list(self._children):
+                ch.close()
             self.lib.aurora_destroy(self.h)
             self.h = None
 
@@ -185,6 +193,7 @@ class Scene:
         ctx.check(self.lib.aurora_scene_upload(ctx.h, C.byref(d), C.byref(h)))
         self.h = h
         self.bvh_stats = None
+        ctx.adopt(self)
 
     def close(self):
         if self.h:
@@ -271,6 +280,8 @@ class Scene:
             self.h, C.byref(c), C.byref(r), rgb.data_ptr(),
             albedo.data_ptr() if albedo is not None else None,
             normal.data_ptr() if normal is not None else None, C.byref(st) if stats else None))
+        if stats:
+            self.ctx.synchronize()
         return stats_dict(st) if stats else None
 
 

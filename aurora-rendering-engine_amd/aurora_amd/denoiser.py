@@ -36,6 +36,7 @@ class Denoiser:
         ctx.check(self.lib.aurora_denoiser_create(ctx.h, seed, C.byref(h)))
         self.h = h
         self.n = int(self.lib.aurora_denoiser_param_count(h))
+        ctx.adopt(self)
 
     def close(self):
         if self.h:
@@ -71,6 +72,7 @@ class Denoiser:
             out = torch.empty((H, W, 3), dtype=torch.float32, device=rgb_sum.device)
         self.ctx.check(self.lib.aurora_denoise(self.h, rgb_sum.data_ptr(), alb_sum.data_ptr(), nrm_sum.data_ptr(),
                                                spp, W, H, out.data_ptr()))
+        self.ctx.synchronize()
         return out
 
     def loss_grad(self, rgb, alb, nrm, target, spp, grads=True):

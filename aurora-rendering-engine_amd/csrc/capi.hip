@@ -511,7 +511,9 @@ int aurora_create(int device, aurora_ctx** out) {
     Status s = hip_status(hipSetDevice(device), "hipSetDevice");
     hipDeviceProp_t prop{};
     if (!s.bad()) s = hip_status(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-    if (!s.bad()) s = hip_status(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    // a blocking stream: ordered with work on the legacy default stream
+    // (where callers such as PyTorch place their copies and fills)
+    if (!s.bad()) s = hip_status(hipStreamCreate(&c->stream), "hipStreamCreate");
     if (s.bad()) {
         delete c;
         return fail(nullptr, s);
