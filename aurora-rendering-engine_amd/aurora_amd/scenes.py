@@ -299,3 +299,24 @@ def random_rays_torch(n, lo, hi, seed=0, device="cuda"):
     d = torch.randn((n, 3), generator=g, device=device)
     d = d / d.norm(dim=1, keepdim=True).clamp_min(1e-12)
     return o.contiguous(), d.contiguous()
+
+
+def cornell_variant(seed):
+    """Denoiser training scene: a Cornell box with box sizes U(100, 200)^3,
+    rotations U(-45, 45) degrees, wall albedos U(0.05, 0.8)^3 and a light of
+    side lengths with area in U(50^2, 200^2).  Draw order: box 1 size (3),
+    box 2 size (3), rotations (2), red, white, green albedos (9), light area
+    (1), light aspect (1)."""
+    rnd = Draws(seed)
+    s1 = tuple(rnd(100, 200) for _ in range(3))
+    s2 = tuple(rnd(100, 200) for _ in range(3))
+    r1, r2 = rnd(-45, 45), rnd(-45, 45)
+    red = tuple(rnd(0.05, 0.8) for _ in range(3))
+    white = tuple(rnd(0.05, 0.8) for _ in range(3))
+    green = tuple(rnd(0.05, 0.8) for _ in range(3))
+    area = rnd(50.0 ** 2, 200.0 ** 2)
+    aspect = rnd(0.6, 1.6)
+    lw = min(500.0, math.sqrt(area * aspect))
+    ld = min(500.0, area / lw)
+    return cornell_box(box_rot=(r1, r2), box_size=(s1, s2), albedo_white=white, albedo_red=red,
+                       albedo_green=green, light_size=(lw, ld))
