@@ -188,6 +188,13 @@ def load_library(path=None):
     if _lib is not None and path is None:
         return _lib
     p = path or LIB_PATH
+    # PyTorch ships its own HIP runtime; importing it first makes that the
+    # one runtime of the process, which libaurora then shares (loading
+    # libaurora first and torch second leaves torch without a device)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(p):
         raise RuntimeError(f"libaurora not built: {p} is missing (run __graft_entry__.build())")
     lib = C.CDLL(p)

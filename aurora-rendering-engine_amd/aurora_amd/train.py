@@ -64,18 +64,23 @@ def make_dataset(ctx, seeds, size, in_spp, target_spp, progress=True):
     return data
 
 
-def train(ctx, data, steps, batch=8, crop=128, lr=2e-3, seed=0, net=None, log_every=100):
+def train(ctx, datasets, steps, batch=8, crop=128, lr=2e-3, seed=0, net=None, log_every=100):
+    """datasets: one dataset or a list; each step draws one dataset
+    uniformly, then one input sample count and `batch` random crops."""
     import torch
 
     from .denoiser import Denoiser
     net = net or Denoiser(ctx, seed=seed)
     g = torch.Generator(device="cpu").manual_seed(seed)
-    n = data["target"].shape[0]
-    size = data["size"]
-    levels = data["in_spp"]
+    if isinstance(datasets, dict):
+        datasets = [datasets]
     t0 = time.time()
     hist = []
     for it in range(steps):
+        data = datasets[int(torch.randint(len(datasets), (1,), generator=g))]
+        n = data["target"].shape[0]
+        size = data["size"]
+        levels = data["in_spp"]
         k = levels[int(torch.randint(len(levels), (1,), generator=g))]
         idx = torch.randint(n, (batch,), generator=g)
         oy = torch.randint(0, size - crop + 1, (batch,), generator=g) // 8 * 8
@@ -130,6 +135,10 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--hires-scenes", type=int, default=0, help="extra variants rendered at --hires-size")
+    ap.add_argument("--hires-size", type=int, default=1024)
+    ap.add_argument("--hires-in-spp", type=str, default="256,1024")
+    ap.add_argument("--hires-target-spp", type=int, default=2048)
     ap.add_argument("--init", type=str, default=None, help="start from these weights")
     ap.add_argument("--out", type=str, required=True)
     ap.add_argument("--report", type=str, default=None)
@@ -141,17 +150,23 @@ def main():
     t0 = time.time()
     data = make_dataset(ctx, list(range(args.scenes)), args.size, in_spp, args.target_spp)
     held = make_dataset(ctx, list(range(100000, 100000 + args.heldout)), args.size, in_spp, args.target_spp)
+    sets = [data]
+    if args.hires_scenes:
+        hin = [int(x) for x in args.hires_in_spp.split(",")]
+        sets.append(make_dataset(ctx, list(range(200000, 200000 + args.hires_scenes)), args.hires_size, hin,
+                                 args.hires_target_spp))
     t_data = time.time() - t0
     net = Denoiser(ctx, seed=0)
     if args.init:
         net.load(args.init)
     before = evaluate(net, held)
     t1 = time.time()
-    net, hist = train(ctx, data, args.steps, args.batch, args.crop, args.lr, net=net)
+    net, hist = train(ctx, sets, args.steps, args.batch, args.crop, args.lr, net=net)
     t_train = time.time() - t1
     net.save(args.out)
     after = evaluate(net, held)
     rep = {"scenes": args.scenes, "size": args.size, "in_spp": in_spp, "target_spp": args.target_spp,
+           "hires_scenes": args.hires_scenes, "hires_size": args.hires_size,
            "steps": args.steps, "batch": args.batch, "crop": args.crop, "dataset_s": t_data, "train_s": t_train,
            "final_loss": float(np.mean(hist[-100:])), "heldout_before": before, "heldout_after": after}
     log(json.dumps(rep))
