@@ -92,6 +92,76 @@ def stress_section(ctx, n_rays=1 << 22, render_spp=1):
             "setup_s": time.perf_counter() - t0}
 
 
+WEIGHTS = os.path.join(ROOT, "aurora-rendering-engine_amd", "weights", "unet_cornell.f32")
+
+
+def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=4096):
+    """Time to reach relMSE <= target on the Cornell box at the bench
+    resolution, plain progressive rendering versus render + denoise.  The
+    reference is two independent FP32 GPU renders of ref_spp samples each
+    (disjoint sample ranges); the reference's own variance is subtracted per
+    pixel (metric.unbiased_relmse)."""
+    import torch
+
+    from aurora_amd.core import RenderSettings
+    from aurora_amd.metric import unbiased_relmse
+
+    H, W = cam.image_height, cam.image_width
+
+    def buf():
+        return torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+
+    def render_into(rgb, spp, offset, alb=None, nrm=None):
+        t0 = time.perf_counter()
+        scene.render_device(cam, RenderSettings(spp=spp, sample_offset=offset, max_depth=50, seed=0,
+                                                aov=alb is not None), rgb, alb, nrm)
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    ra, rb = buf(), buf()
+    render_into(ra, ref_spp, 1 << 28)
+    render_into(rb, ref_spp, (1 << 28) + ref_spp)
+    A, B = ra / ref_spp, rb / ref_spp
+    out = {"target_relmse": target, "resolution": [W, H],
+           "reference": f"2 x {ref_spp} spp GPU FP32 halves, per-pixel variance subtracted"}
+    acc = buf()
+    done, t_plain, curve = 0, 0.0, []
+    err = float("inf")
+    for cp in [1024 * k for k in (1, 2, 4, 8, 12, 16, 20, 24, 28, 32, 40, 48)]:
+        t_plain += render_into(acc, cp - done, done)
+        done = cp
+        err = unbiased_relmse(acc / done, A, B)
+        curve.append([done, err])
+        if err <= target:
+            break
+    out["plain"] = {"spp": done, "seconds": t_plain, "relmse": err, "reached": err <= target, "curve": curve}
+    if os.path.exists(WEIGHTS):
+        from aurora_amd.denoiser import Denoiser
+        dn = Denoiser(ctx)
+        dn.load(WEIGHTS)
+        dcurve = []
+        best = None
+        for spp in (64, 128, 256, 512, 1024, 2048, 4096):
+            rgb, alb, nrm = buf(), buf(), buf()
+            tr = render_into(rgb, spp, 0, alb, nrm)
+            t0 = time.perf_counter()
+            img = dn.denoise(rgb, alb, nrm, spp)
+            td = time.perf_counter() - t0
+            e = unbiased_relmse(img, A, B)
+            e_plain = unbiased_relmse(rgb / spp, A, B)
+            dcurve.append([spp, e, e_plain, tr, td])
+            if e <= target:
+                best = {"spp": spp, "seconds": tr + td, "render_s": tr, "denoise_s": td, "relmse": e}
+                break
+        out["denoised"] = best or {"reached": False}
+        out["denoised_curve"] = dcurve  # spp, relMSE denoised, relMSE plain, render s, denoise s
+        if best and out["plain"]["reached"]:
+            out["speedup"] = out["plain"]["seconds"] / best["seconds"]
+    else:
+        out["denoised"] = None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +173,7 @@ def main():
     ap.add_argument("--tail-paths", type=int, default=None, help="fused-tail switch (default: library's)")
     ap.add_argument("--sweep-tail", action="store_true", help="time several fused-tail switch points")
     ap.add_argument("--no-stress", action="store_true", help="skip the traversal stress scene")
+    ap.add_argument("--no-metric", action="store_true", help="skip time-to-relMSE (plain vs denoised)")
     ap.add_argument("--only", choices=["stress", "copy"], default=None,
                     help="profiling modes: the stress section alone, or a plain 1 GiB device copy "
                          "(calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -205,6 +276,8 @@ def main():
                      "kernel": "k_extend", "alg_bytes_per_launch": alg_bytes / max(1, first["extend_launches"]),
                      "avg_launch_ms": first["extend_ms"] / max(1, first["extend_launches"])},
     }
+    if not args.no_metric:
+        out["time_to_relmse"] = time_to_relmse_section(ctx, scene, cam)
     if not args.no_stress:
         scene.close()
         out["stress"] = stress_section(ctx)
