@@ -27,6 +27,7 @@ TEX_IMAGE = 3
 
 RENDER_AOV = 1
 RENDER_COUNT_FETCHES = 2
+RENDER_GRAPH = 4
 
 NODE_LEAF = 0x80000000
 MISS = -1
@@ -117,6 +118,11 @@ class RenderDesc(C.Structure):
     ]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", CameraDesc), ("render", RenderDesc), ("d_rgb_sum", C.c_void_p),
+                ("d_albedo_sum", C.c_void_p), ("d_normal_sum", C.c_void_p)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [
         ("paths", C.c_uint64),
@@ -163,6 +169,7 @@ EXPORTS = [
     ("aurora_render_device", C.c_int,
      [_vp, C.POINTER(CameraDesc), C.POINTER(RenderDesc), _vp, _vp, _vp, C.POINTER(RenderStats)]),
     ("aurora_resolve_rgb8", C.c_int, [_vp, _u32, _u32, _u32, _vp]),
+    ("aurora_render_batch_device", C.c_int, [_vp, C.POINTER(BatchItem), _u32, _u32, C.POINTER(RenderStats)]),
     ("aurora_denoiser_create", C.c_int, [_vp, _u32, C.POINTER(_vp)]),
     ("aurora_denoiser_destroy", None, [_vp]),
     ("aurora_denoiser_param_count", C.c_uint64, [_vp]),

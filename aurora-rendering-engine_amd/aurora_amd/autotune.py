@@ -1,0 +1,71 @@
+"""Launch-shape autotuner.
+
+A workload (one scene or a batch) is rendered once per candidate launch
+shape, and the fastest is kept and cached under a signature of the
+workload (pixels, samples, primitives).  The shape covers the persistent
+traversal and shading grids (blocks per compute unit), the live-path count
+at which the fused tail takes over, and whether passes are captured in a
+hipGraph.  Shapes change speed only, never results (tests check the images
+bit for bit), so the tuner can be run at any time.
+"""
+import itertools
+import json
+import os
+import time
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "aurora_amd", "autotune.json")
+
+CANDIDATES = {
+    "extend_blocks_per_cu": [4, 8, 16],
+    "shade_blocks_per_cu": [8],
+    "tail_paths": [1 << 18, 1 << 20, 1 << 22],
+    "graph": [False, True],
+}
+
+
+def signature(items):
+    """Workload signature from (scene, camera, settings) triples."""
+    px = sum(c.image_width * c.image_height for _, c, _ in items)
+    spp = sum(c.image_width * c.image_height * rs.spp for _, c, rs in items)
+    prims = sum(sc.arrays.n_prims for sc, _, _ in items)
+    return f"n{len(items)}_px{px}_paths{spp}_prims{prims}"
+
+
+def tune(ctx, run, sig, cache_path=DEFAULT_CACHE, candidates=None, repeats=2, log=None):
+    """run(graph) renders the workload once.  Returns (best shape, table of
+    timings); the best shape is left applied to ctx."""
+    cache = {}
+    if cache_path and os.path.exists(cache_path):
+        try:
+            with open(cache_path) as f:
+                cache = json.load(f)
+        except (OSError, ValueError):
+            cache = {}
+    if sig in cache:
+        best = cache[sig]["best"]
+        ctx.set_launch(best["extend_blocks_per_cu"], best["shade_blocks_per_cu"], best["tail_paths"])
+        return best, cache[sig]["table"]
+    cand = candidates or CANDIDATES
+    keys = list(cand)
+    table = []
+    for values in itertools.product(*(cand[k] for k in keys)):
+        shape = dict(zip(keys, values))
+        ctx.set_launch(shape["extend_blocks_per_cu"], shape["shade_blocks_per_cu"], shape["tail_paths"])
+        run(shape["graph"])  # warm-up (allocations, graph capture paths)
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            run(shape["graph"])
+            ts.append(time.perf_counter() - t0)
+        shape["seconds"] = min(ts)
+        table.append(shape)
+        if log:
+            log(f"autotune {sig}: {shape}")
+    best = min(table, key=lambda s: s["seconds"])
+    ctx.set_launch(best["extend_blocks_per_cu"], best["shade_blocks_per_cu"], best["tail_paths"])
+    if cache_path:
+        cache[sig] = {"best": best, "table": table}
+        os.makedirs(os.path.dirname(cache_path), exist_ok=True)
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1)
+    return best, table

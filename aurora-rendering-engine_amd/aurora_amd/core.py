@@ -117,6 +117,7 @@ class RenderSettings:
     background: tuple = (0.0, 0.0, 0.0)
     aov: bool = False
     count_fetches: bool = False
+    graph: bool = False
     spp_per_pass: int = 0
     tmin: float = 0.0
 
@@ -127,7 +128,8 @@ class RenderSettings:
         r.max_depth = self.max_depth
         r.seed = self.seed
         r.background = (C.c_float * 3)(*self.background)
-        r.flags = (abi.RENDER_AOV if self.aov else 0) | (abi.RENDER_COUNT_FETCHES if self.count_fetches else 0)
+        r.flags = ((abi.RENDER_AOV if self.aov else 0) | (abi.RENDER_COUNT_FETCHES if self.count_fetches else 0)
+                   | (abi.RENDER_GRAPH if self.graph else 0))
         r.spp_per_pass = self.spp_per_pass
         r.tmin = self.tmin
         return r
@@ -283,6 +285,25 @@ class Scene:
         if stats:
             self.ctx.synchronize()
         return stats_dict(st) if stats else None
+
+
+def render_batch(ctx, items, graph=False):
+    """Render many scenes in shared passes.  items: (scene, camera,
+    settings, rgb, albedo, normal) with device tensors (albedo/normal may be
+    None).  Returns the stats dict; synchronous."""
+    arr = (abi.BatchItem * max(1, len(items)))()
+    for k, (sc, cam, rs, rgb, alb, nrm) in enumerate(items):
+        arr[k].scene = sc.h
+        arr[k].camera = cam.desc()
+        arr[k].render = rs.desc()
+        arr[k].d_rgb_sum = rgb.data_ptr()
+        arr[k].d_albedo_sum = alb.data_ptr() if alb is not None else None
+        arr[k].d_normal_sum = nrm.data_ptr() if nrm is not None else None
+    st = abi.RenderStats()
+    ctx.check(ctx.lib.aurora_render_batch_device(ctx.h, arr, len(items), abi.RENDER_GRAPH if graph else 0,
+                                                 C.byref(st)))
+    ctx.synchronize()
+    return stats_dict(st)
 
 
 def resolve_rgb8(rgb_sum, spp):

@@ -320,3 +320,48 @@ def cornell_variant(seed):
     ld = min(500.0, area / lw)
     return cornell_box(box_rot=(r1, r2), box_size=(s1, s2), albedo_white=white, albedo_red=red,
                        albedo_green=green, light_size=(lw, ld))
+
+
+def batch_scene(seed, min_prims=10, max_prims=200_000):
+    """Batched-serving scene: 10 * (2e4)^U primitives (spheres and quads in
+    a random ratio) scattered in [-1, 1]^3 with lambertian / metal / glass
+    materials and checker / noise textures, under a sky; resolution from
+    {256, 384, 512, 768, 1024}^2 and 1..16 spp.  Returns (arrays, camera,
+    spp)."""
+    rnd = Draws(seed)
+    n = int(min(max_prims, max(min_prims, round(10 * (2e4) ** rnd()))))
+    frac_sph = rnd()
+    res = [256, 384, 512, 768, 1024][min(4, int(rnd() * 5))]
+    spp = 1 + min(15, int(rnd() * 16))
+    b = Builder()
+    t0 = b.texture(solid(0.8, 0.3, 0.2))
+    t1 = b.texture(solid(0.9, 0.9, 0.9))
+    chk = b.texture(checker(0.25, t0, t1))
+    mats = [b.material(abi.MAT_LAMBERTIAN, chk), b.lambertian((0.4, 0.6, 0.8)),
+            b.material(abi.MAT_LAMBERTIAN, b.texture(noise(4.0, seed=seed))), b.metal((0.8, 0.8, 0.7), 0.1),
+            b.dielectric(1.5)]
+    ns = int(n * frac_sph)
+    nq = n - ns
+    # vectorised draws keep large scenes quick to build
+    rnd.k += (-rnd.k) % 4
+    rnd.buf = []
+    scale = 1.0 / max(1.0, n ** (1.0 / 3.0))
+    if ns:
+        c = rnd.array(3 * ns, -1.0, 1.0).reshape(-1, 3)
+        r = rnd.array(ns, 0.2, 1.0) * 0.5 * scale
+        m = (rnd.array(ns) * len(mats)).astype(np.int64).clip(0, len(mats) - 1)
+        b.spheres.extend(map(tuple, np.concatenate([c, r[:, None]], 1)))
+        b.sphere_mat.extend(int(mats[i]) for i in m)
+    if nq:
+        Q = rnd.array(3 * nq, -1.0, 1.0).reshape(-1, 3)
+        u = rnd.array(3 * nq, -1.0, 1.0).reshape(-1, 3) * scale
+        v = rnd.array(3 * nq, -1.0, 1.0).reshape(-1, 3) * scale
+        bad = np.linalg.norm(np.cross(u, v), axis=1) < 1e-6 * scale * scale
+        u[bad] = np.array([scale, 0, 0])
+        v[bad] = np.array([0, scale, 0])
+        m = (rnd.array(nq) * len(mats)).astype(np.int64).clip(0, len(mats) - 1)
+        b.quads.extend(map(tuple, np.concatenate([Q, u, v], 1)))
+        b.quad_mat.extend(int(mats[i]) for i in m)
+        b.quad_kind.extend([abi.QUAD_PARALLELOGRAM] * nq)
+    cam = Camera(res, res, 40.0, (0.0, 0.5, 4.0), (0.0, 0.0, 0.0))
+    return b.arrays(), cam, spp

@@ -28,6 +28,10 @@ struct aurora_ctx {
     int ext_grid = 0;    // resident blocks of the extend kernel
     int shade_grid = 0;
     uint32_t tail_paths = 1u << 20;  // fused-tail switch: live paths at or below
+    uint32_t graph_tail_depth = 8;   // captured passes switch to the fused tail here
+    Workspace ws;                    // path queues shared by the context's renders
+    SceneRec* host_recs = nullptr;   // pinned staging of the pass's scene table
+    uint32_t host_recs_cap = 0;
 };
 
 struct aurora_scene {
@@ -39,7 +43,7 @@ struct aurora_scene {
     BvhScratch scr;
     bool built = false;
     aurora_bvh_stats stats{};
-    Workspace ws;
+    DevBuf rec1;  // one-entry scene table for ray queries
     DevBuf tr_o, tr_d, tr_hit, tr_spill, tr_pool, totals;
 };
 
@@ -250,6 +254,13 @@ Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n,
     } else {
         const int grid = grid_for_extend(c, n);
         AUR_TRY(ensure_spill(s->tr_spill, grid));
+        // the query's ray offset travels in the one-entry scene table
+        SceneRec rec{};
+        rec.sc = s->view;
+        rec.tmin = tmin;
+        AUR_TRY(s->rec1.alloc(sizeof(SceneRec)));
+        AUR_HIP(hipStreamSynchronize(c->stream));
+        AUR_HIP(hipMemcpy(s->rec1.p, &rec, sizeof(SceneRec), hipMemcpyHostToDevice));
         AUR_TRY(s->tr_pool.alloc(kPools * kPoolStride * 4));
         AUR_HIP(hipMemsetAsync(s->tr_pool.p, 0, kPools * kPoolStride * 4, c->stream));
         ExtendArgs a{};
@@ -258,7 +269,6 @@ Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n,
         a.count = nullptr;
         a.n = n;
         a.pool = s->tr_pool.as<uint32_t>();
-        a.tmin = tmin;
         a.tmax = tmax;
         a.hit = s->tr_hit.as<float2>();
         a.counts = counts;
@@ -275,7 +285,8 @@ Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n,
             AUR_HIP(hipEventCreate(&e1));
             AUR_HIP(hipEventRecord(e0, c->stream));
         }
-        Status ls = launch_extend(s->view, a, grid, c->stream);
+        a.recs = s->rec1.as<SceneRec>();
+        Status ls = launch_extend(a, grid, c->stream);
         if (ts) {
             Status es = hip_status(hipEventRecord(e1, c->stream), "hipEventRecord");
             if (!es.bad()) es = hip_status(hipEventSynchronize(e1), "hipEventSynchronize");
@@ -302,8 +313,8 @@ Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n,
     return Status::ok();
 }
 
-Status ensure_workspace(aurora_scene* s, uint32_t cap, int ext_grid) {
-    Workspace& w = s->ws;
+Status ensure_workspace(aurora_ctx* c, uint32_t cap, int grid) {
+    Workspace& w = c->ws;
     if (cap > w.capacity) {
         for (int k = 0; k < 2; ++k) {
             AUR_TRY(w.q_o[k].alloc((size_t)cap * 16));
@@ -317,20 +328,29 @@ Status ensure_workspace(aurora_scene* s, uint32_t cap, int ext_grid) {
         w.capacity = cap;
     }
     AUR_TRY(w.counters.alloc(kCounterBytes));
-    AUR_TRY(ensure_spill(w.spill, ext_grid));
+    AUR_TRY(ensure_spill(w.spill, grid));
     if (!w.host_counts) AUR_HIP(hipHostMalloc((void**)&w.host_counts, 256 * sizeof(uint32_t), hipHostMallocDefault));
     return Status::ok();
 }
 
 constexpr uint32_t kPassPaths = 1u << 23;  // wavefront width per pass
 
-Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const aurora_render_desc* rd, float* rgb,
-                     float* alb, float* nrm, aurora_render_stats* stats, bool count_fetches) {
-    aurora_ctx* c = s->ctx;
+// One image of a render call.
+struct Job {
+    aurora_scene* scene;
+    CamF cam;
+    aurora_render_desc rd;
+    float *rgb, *alb, *nrm;
+    uint32_t S;     // samples per pass
+    uint32_t done;  // samples rendered so far
+};
+
+Status validate_job(aurora_scene* s, const aurora_camera_desc* cam, const aurora_render_desc* rd, float* rgb,
+                    float* alb, float* nrm, Job& j) {
+    if (!s) return Status{AURORA_ERR_INVALID, "scene is NULL"};
     if (!cam || !rd) return Status{AURORA_ERR_INVALID, "null camera or render descriptor"};
     if (!rgb) return Status{AURORA_ERR_INVALID, "rgb_sum is NULL"};
-    RenderArgs r{};
-    const std::string e = camera_frame(*cam, r.cam);
+    const std::string e = camera_frame(*cam, j.cam);
     if (!e.empty()) return Status{AURORA_ERR_INVALID, e};
     if (rd->max_depth == 0) return Status{AURORA_ERR_INVALID, "max_depth must be >= 1"};
     if (rd->max_depth > 250) return Status{AURORA_ERR_INVALID, "max_depth must be <= 250"};
@@ -339,34 +359,58 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
         return Status{AURORA_ERR_INVALID, "AOV flag set but albedo/normal buffers are NULL"};
     if (!s->built) return Status{AURORA_ERR_STATE, "BVH not built: call aurora_bvh_build first"};
     if ((uint64_t)rd->sample_offset + rd->spp > 0xffffffffull) return Status{AURORA_ERR_INVALID, "sample index overflow"};
+    j.scene = s;
+    j.rd = *rd;
+    j.rgb = rgb;
+    j.alb = alb;
+    j.nrm = nrm;
+    j.done = 0;
+    return Status::ok();
+}
+
+// Render every job's samples in wavefront passes that share launches.
+Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* stats, bool count_fetches,
+                   bool graph) {
     aurora_render_stats st{};
-    const uint32_t WH = r.cam.W * r.cam.H;
-    uint32_t S = rd->spp_per_pass;
-    if (S == 0) S = std::max<uint32_t>(1, kPassPaths / WH);
-    S = std::min<uint32_t>(S, std::max<uint32_t>(1, rd->spp));
-    if ((uint64_t)S * WH > 0xffffffffull) return Status{AURORA_ERR_INVALID, "spp_per_pass * pixels exceeds 2^32"};
-    if (rd->spp == 0) {
+    uint64_t total_wh = 0;
+    for (const Job& j : jobs) total_wh += (uint64_t)j.cam.W * j.cam.H;
+    // samples per pass: explicit, or an equal share of the pass width
+    for (Job& j : jobs) {
+        const uint64_t wh = (uint64_t)j.cam.W * j.cam.H;
+        uint64_t S = j.rd.spp_per_pass ? j.rd.spp_per_pass : std::max<uint64_t>(1, kPassPaths / std::max<uint64_t>(1, total_wh));
+        S = std::min<uint64_t>(S, std::max<uint32_t>(1, j.rd.spp));
+        if (S * wh > 0xffffffffull) return Status{AURORA_ERR_INVALID, "spp_per_pass * pixels exceeds 2^32"};
+        j.S = (uint32_t)S;
+    }
+    uint64_t cap = 0;
+    for (const Job& j : jobs) cap += (uint64_t)j.S * j.cam.W * j.cam.H;
+    if (cap > 0x7fffffffull) return Status{AURORA_ERR_INVALID, "a pass would exceed 2^31 paths"};
+    uint32_t max_depth = 1;
+    bool any_work = false;
+    for (const Job& j : jobs) {
+        max_depth = std::max(max_depth, j.rd.max_depth);
+        any_work |= j.rd.spp > 0;
+    }
+    if (!any_work) {
         if (stats) *stats = st;
         return Status::ok();
     }
     const int ext_grid = c->ext_grid;
-    // the fused tail runs one path per thread on up to 4x the resident grid
-    const int tail_grid = (int)std::min<uint64_t>(4ull * ext_grid,
-                                                  std::max<uint64_t>(1, ((uint64_t)std::min<uint64_t>(
-                                                                            c->tail_paths, (uint64_t)S * WH) +
-                                                                        255) / 256));
-    AUR_TRY(ensure_workspace(s, S * WH, std::max(ext_grid, tail_grid)));
-    Workspace& w = s->ws;
-    r.seed = rd->seed;
-    r.max_depth = rd->max_depth;
-    r.tmin = rd->tmin > 0.0f ? rd->tmin : 0.001f;
-    r.bg = V(rd->background[0], rd->background[1], rd->background[2]);
-    r.aov = (rd->flags & AURORA_RENDER_AOV) != 0;
-    if (count_fetches) {
-        AUR_TRY(s->totals.alloc(64));
-        AUR_HIP(hipMemsetAsync(s->totals.p, 0, 24, c->stream));
+    const int tail_grid = (int)std::min<uint64_t>(4ull * ext_grid, std::max<uint64_t>(1, (std::min<uint64_t>(c->tail_paths, cap) + 255) / 256));
+    const int graph_tail_grid = 4 * ext_grid;
+    AUR_TRY(ensure_workspace(c, (uint32_t)cap, std::max(ext_grid, graph ? graph_tail_grid : tail_grid)));
+    Workspace& w = c->ws;
+    AUR_TRY(w.recs.alloc(jobs.size() * sizeof(SceneRec)));
+    if (c->host_recs_cap < jobs.size()) {
+        if (c->host_recs) (void)hipHostFree(c->host_recs);
+        c->host_recs = nullptr;
+        AUR_HIP(hipHostMalloc((void**)&c->host_recs, jobs.size() * sizeof(SceneRec), hipHostMallocDefault));
+        c->host_recs_cap = (uint32_t)jobs.size();
     }
-
+    if (count_fetches) {
+        AUR_TRY(jobs[0].scene->totals.alloc(64));
+        AUR_HIP(hipMemsetAsync(jobs[0].scene->totals.p, 0, 24, c->stream));
+    }
     const bool timed = stats != nullptr;
     std::vector<hipEvent_t> evs;
     auto ev = [&](hipEvent_t& e) -> Status {
@@ -374,32 +418,144 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
         evs.push_back(e);
         return Status::ok();
     };
+    struct Cleanup {
+        std::vector<hipEvent_t>& e;
+        ~Cleanup() {
+            for (auto& x : e) (void)hipEventDestroy(x);
+        }
+    } cleanup{evs};
     hipEvent_t t_begin = nullptr, t_end = nullptr;
     if (timed) {
         AUR_TRY(ev(t_begin));
         AUR_TRY(ev(t_end));
         AUR_HIP(hipEventRecord(t_begin, c->stream));
     }
-    std::vector<hipEvent_t> ext_ev;  // begin/end per extend launch
-    std::vector<hipEvent_t> cnt_ev(rd->max_depth);
+    std::vector<hipEvent_t> ext_ev;
+    std::vector<hipEvent_t> cnt_ev(max_depth);
     for (auto& e2 : cnt_ev) AUR_TRY(ev(e2));
-    Status result = Status::ok();
-    for (uint32_t done = 0; done < rd->spp && !result.bad(); done += S) {
-        const uint32_t sp = std::min(S, rd->spp - done);
-        r.spp_pass = sp;
-        r.sample_base = rd->sample_offset + done;
-        const uint32_t n = sp * WH;
+    uint64_t total_segments = 0;
+
+    // build the scene table of the next pass; returns its shape
+    auto make_pass = [&](Pass& pa, bool& same_shape, uint32_t& prev_sig) -> bool {
+        uint32_t path_off = 0, pix_off = 0, sig = 0;
+        bool any = false;
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            Job& j = jobs[k];
+            const uint32_t sp = j.done < j.rd.spp ? std::min(j.S, j.rd.spp - j.done) : 0u;
+            SceneRec& r = c->host_recs[k];
+            r.sc = j.scene->view;
+            r.cam = j.cam;
+            r.seed = j.rd.seed;
+            r.sample_base = j.rd.sample_offset + j.done;
+            r.spp_pass = sp;
+            r.max_depth = j.rd.max_depth;
+            r.tmin = j.rd.tmin > 0.0f ? j.rd.tmin : 0.001f;
+            for (int q = 0; q < 3; ++q) r.bg[q] = j.rd.background[q];
+            r.aov = (j.rd.flags & AURORA_RENDER_AOV) ? 1u : 0u;
+            r.WH = j.cam.W * j.cam.H;
+            r.path_off = path_off;
+            r.pix_off = pix_off;
+            r.rgb = j.rgb;
+            r.alb = j.alb;
+            r.nrm = j.nrm;
+            path_off += r.WH * sp;
+            if (sp) pix_off += r.WH;
+            any |= sp > 0;
+            sig = sig * 1000003u + sp;
+        }
+        pa.recs = w.recs.as<SceneRec>();
+        pa.n_recs = (uint32_t)jobs.size();
+        pa.n_paths = path_off;
+        pa.n_pixels = pix_off;
+        same_shape = sig == prev_sig;
+        prev_sig = sig;
+        return any;
+    };
+    auto finish_pass = [&](const Pass& pa) {
+        for (Job& j : jobs) j.done += j.done < j.rd.spp ? std::min(j.S, j.rd.spp - j.done) : 0u;
         st.passes++;
-        st.paths += n;
-        st.max_live = std::max(st.max_live, n);
-        AUR_TRY(launch_raygen(s->view, r, w, n, c->stream));
-        uint32_t launched = 0;
-        bool tail = false;
-        for (uint32_t depth = 0; depth < rd->max_depth; ++depth) {
+        st.paths += pa.n_paths;
+        st.max_live = std::max(st.max_live, pa.n_paths);
+    };
+    auto read_segments = [&]() -> Status {
+        unsigned long long v = 0;
+        AUR_HIP(hipMemcpyAsync(&v, w.counters.as<uint32_t>() + kTailSegs, 8, hipMemcpyDeviceToHost, c->stream));
+        AUR_HIP(hipStreamSynchronize(c->stream));
+        total_segments += v;
+        return Status::ok();
+    };
+    auto extend_args = [&](int cur) {
+        ExtendArgs a{};
+        a.ray_o = w.q_o[cur].as<float4>();
+        a.ray_d = w.q_d[cur].as<float4>();
+        a.count = w.counters.as<uint32_t>() + cur;
+        a.n = 0;
+        a.pool = w.counters.as<uint32_t>() + kPoolBase;
+        a.zero_after = w.counters.as<uint32_t>() + (cur ^ 1);
+        a.tmax = INFINITY;
+        a.hit = w.hit.as<float2>();
+        a.recs = w.recs.as<SceneRec>();
+        a.totals = count_fetches ? jobs[0].scene->totals.as<unsigned long long>() : nullptr;
+        a.spill = w.spill.as<uint32_t>();
+        a.spill_stride = (uint32_t)(std::max(ext_grid, graph ? graph_tail_grid : tail_grid) * extend_block());
+        return a;
+    };
+
+    uint32_t prev_sig = 0xffffffffu;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t g = nullptr;
+    for (;;) {
+        Pass pa{};
+        bool same = false;
+        if (!make_pass(pa, same, prev_sig)) break;
+        if (graph && same && gexec) {
+            // same pass shape as the captured one: replay (the graph's last
+            // kernel advanced the device table's sample bases)
+            AUR_HIP(hipGraphLaunch(gexec, c->stream));
+            AUR_TRY(read_segments());
+            finish_pass(pa);
+            continue;
+        }
+        AUR_HIP(hipMemcpyAsync(w.recs.p, c->host_recs, jobs.size() * sizeof(SceneRec), hipMemcpyHostToDevice,
+                               c->stream));
+        if (graph) {
+            // capture: fixed launch shapes, no host round trips inside
+            if (gexec) {
+                (void)hipGraphExecDestroy(gexec);
+                gexec = nullptr;
+            }
+            if (g) {
+                (void)hipGraphDestroy(g);
+                g = nullptr;
+            }
+            AUR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            Status cs = launch_raygen(pa, w, c->stream);
+            const uint32_t dt = std::min(max_depth, c->graph_tail_depth);
+            for (uint32_t depth = 0; depth < dt && !cs.bad(); ++depth) {
+                const int cur = depth & 1;
+                ExtendArgs a = extend_args(cur);
+                a.spill_stride = (uint32_t)(graph_tail_grid * extend_block());
+                cs = launch_extend(a, ext_grid, c->stream);
+                if (!cs.bad()) cs = launch_shade(pa, w, cur, depth, c->shade_grid, c->stream);
+            }
+            if (!cs.bad() && dt < max_depth) cs = launch_tail(pa, w, dt & 1, dt, graph_tail_grid, c->stream);
+            if (!cs.bad()) cs = launch_accumulate(pa, w, c->stream);
+            if (!cs.bad()) cs = launch_advance(w.recs.as<SceneRec>(), pa.n_recs, c->stream);
+            hipError_t ce = hipStreamEndCapture(c->stream, &g);
+            AUR_TRY(cs);
+            AUR_HIP(ce);
+            AUR_HIP(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0));
+            AUR_HIP(hipGraphLaunch(gexec, c->stream));
+            AUR_TRY(read_segments());
+            finish_pass(pa);
+            continue;
+        }
+        AUR_TRY(launch_raygen(pa, w, c->stream));
+        for (uint32_t depth = 0; depth < max_depth; ++depth) {
             const int cur = depth & 1;
             // queues only shrink, so the count two bounces back bounds this
             // bounce's queue: size both launches from it
-            uint32_t ub = n;
+            uint32_t ub = pa.n_paths;
             if (depth >= 2) {
                 AUR_HIP(hipEventSynchronize(cnt_ev[depth - 2]));
                 ub = w.host_counts[depth - 2];
@@ -409,56 +565,36 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
             const uint64_t blocks = std::max<uint64_t>(1, ((uint64_t)ub + 255) / 256);
             if (ub <= c->tail_paths) {
                 const int tg = (int)std::min<uint64_t>((uint64_t)tail_grid, blocks);
-                AUR_TRY(launch_tail(s->view, r, w, cur, depth, tg, c->stream));
-                tail = true;
+                AUR_TRY(launch_tail(pa, w, cur, depth, tg, c->stream));
                 break;
             }
             const int eg = (int)std::min<uint64_t>((uint64_t)ext_grid, blocks);
             const int sg = (int)std::min<uint64_t>((uint64_t)c->shade_grid, blocks);
-            ExtendArgs a{};
-            a.ray_o = w.q_o[cur].as<float4>();
-            a.ray_d = w.q_d[cur].as<float4>();
-            a.count = w.counters.as<uint32_t>() + cur;
-            a.n = 0;
-            a.pool = w.counters.as<uint32_t>() + kPoolBase;
-            a.zero_after = w.counters.as<uint32_t>() + (cur ^ 1);
-            a.tmin = r.tmin;
-            a.tmax = INFINITY;
-            a.hit = w.hit.as<float2>();
-            a.totals = count_fetches ? s->totals.as<unsigned long long>() : nullptr;
-            a.spill = w.spill.as<uint32_t>();
-            a.spill_stride = (uint32_t)(ext_grid * extend_block());
+            ExtendArgs a = extend_args(cur);
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (timed) {
                 AUR_TRY(ev(e0));
                 AUR_TRY(ev(e1));
                 AUR_HIP(hipEventRecord(e0, c->stream));
             }
-            AUR_TRY(launch_extend(s->view, a, eg, c->stream));
+            AUR_TRY(launch_extend(a, eg, c->stream));
             if (timed) {
                 AUR_HIP(hipEventRecord(e1, c->stream));
                 ext_ev.push_back(e0);
                 ext_ev.push_back(e1);
             }
-            AUR_TRY(launch_shade(s->view, r, w, cur, depth, sg, c->stream));
+            AUR_TRY(launch_shade(pa, w, cur, depth, sg, c->stream));
             AUR_HIP(hipMemcpyAsync(w.host_counts + depth, w.counters.as<uint32_t>() + (cur ^ 1), 4,
                                    hipMemcpyDeviceToHost, c->stream));
             AUR_HIP(hipEventRecord(cnt_ev[depth], c->stream));
-            launched = depth + 1;
         }
-        AUR_TRY(launch_accumulate(r, w, rgb, alb, nrm, c->stream));
-        unsigned long long tail_segs = 0;
-        if (tail)
-            AUR_HIP(hipMemcpyAsync(&tail_segs, w.counters.as<uint32_t>() + kTailSegs, 8, hipMemcpyDeviceToHost,
-                                   c->stream));
-        AUR_HIP(hipStreamSynchronize(c->stream));
-        // queue sizes: bounce 0 holds n paths, bounce j >= 1 host_counts[j-1];
-        // the tail counted its own segments
-        uint64_t segs = launched > 0 ? n : 0;
-        const uint32_t counted = tail ? (launched > 0 ? launched - 1 : 0) : launched;
-        for (uint32_t k = 0; k < counted; ++k) segs += w.host_counts[k];
-        st.segments += segs + tail_segs;
+        AUR_TRY(launch_accumulate(pa, w, c->stream));
+        AUR_TRY(read_segments());
+        finish_pass(pa);
     }
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (g) (void)hipGraphDestroy(g);
+    st.segments = total_segments;
     if (timed) {
         AUR_HIP(hipEventRecord(t_end, c->stream));
         AUR_HIP(hipEventSynchronize(t_end));
@@ -476,15 +612,21 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
     }
     if (count_fetches) {
         unsigned long long tot[3];
-        AUR_HIP(hipMemcpyAsync(tot, s->totals.p, 24, hipMemcpyDeviceToHost, c->stream));
+        AUR_HIP(hipMemcpyAsync(tot, jobs[0].scene->totals.p, 24, hipMemcpyDeviceToHost, c->stream));
         AUR_HIP(hipStreamSynchronize(c->stream));
         st.nodes_fetched = tot[0];
         st.spheres_fetched = tot[1];
         st.quads_fetched = tot[2];
     }
-    for (auto& e2 : evs) (void)hipEventDestroy(e2);
     if (stats) *stats = st;
-    return result;
+    return Status::ok();
+}
+
+Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const aurora_render_desc* rd, float* rgb,
+                     float* alb, float* nrm, aurora_render_stats* stats, bool count_fetches) {
+    std::vector<Job> jobs(1);
+    AUR_TRY(validate_job(s, cam, rd, rgb, alb, nrm, jobs[0]));
+    return render_jobs(s->ctx, jobs, stats, count_fetches, (rd->flags & AURORA_RENDER_GRAPH) != 0);
 }
 
 }  // namespace
@@ -527,10 +669,14 @@ int aurora_create(int device, aurora_ctx** out) {
 
 void aurora_destroy(aurora_ctx* c) {
     if (!c) return;
-    if (c->stream) {
-        (void)hipStreamSynchronize(c->stream);
-        (void)hipStreamDestroy(c->stream);
-    }
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    Workspace& w = c->ws;
+    DevBuf* bufs[] = {&w.q_o[0], &w.q_o[1], &w.q_d[0], &w.q_d[1], &w.q_b[0], &w.q_b[1], &w.hit, &w.L,
+                      &w.aov_a, &w.aov_n, &w.counters, &w.spill, &w.recs};
+    for (DevBuf* b : bufs) b->release();
+    if (w.host_counts) (void)hipHostFree(w.host_counts);
+    if (c->host_recs) (void)hipHostFree(c->host_recs);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -627,12 +773,9 @@ void aurora_scene_destroy(aurora_scene* s) {
                       &s->perlin_perm, &s->nodes, &s->morton, &s->order, &s->scr.box_lo, &s->scr.box_hi,
                       &s->scr.bounds, &s->scr.keys[0], &s->scr.keys[1], &s->scr.vals[0], &s->scr.vals[1],
                       &s->scr.hist, &s->scr.gamma, &s->scr.flags, &s->scr.int_parent, &s->scr.leaf_parent,
-                      &s->scr.int_pos, &s->scr.leaf_pos, &s->scr.arrive, &s->scr.depth, &s->ws.q_o[0],
-                      &s->ws.q_o[1], &s->ws.q_d[0], &s->ws.q_d[1], &s->ws.q_b[0], &s->ws.q_b[1], &s->ws.hit,
-                      &s->ws.L, &s->ws.aov_a, &s->ws.aov_n, &s->ws.counters, &s->ws.spill, &s->tr_o, &s->tr_d,
+                      &s->scr.int_pos, &s->scr.leaf_pos, &s->scr.arrive, &s->scr.depth, &s->rec1, &s->tr_o, &s->tr_d,
                       &s->tr_hit, &s->tr_spill, &s->tr_pool, &s->totals};
     for (DevBuf* b : bufs) b->release();
-    if (s->ws.host_counts) (void)hipHostFree(s->ws.host_counts);
     delete s;
 }
 
@@ -745,6 +888,27 @@ int aurora_render_device(aurora_scene* s, const aurora_camera_desc* cam, const a
     CK(s->ctx, hip_status(hipSetDevice(s->ctx->device), "hipSetDevice"));
     const bool count = rd && (rd->flags & AURORA_RENDER_COUNT_FETCHES);
     CK(s->ctx, render_device(s, cam, rd, rgb, alb, nrm, stats, count));
+    return AURORA_OK;
+}
+
+int aurora_render_batch_device(aurora_ctx* c, const aurora_batch_item* items, uint32_t n, uint32_t flags,
+                               aurora_render_stats* stats) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (n == 0) {
+        if (stats) *stats = aurora_render_stats{};
+        return AURORA_OK;
+    }
+    if (!items) return fail(c, AURORA_ERR_INVALID, "items is NULL");
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    std::vector<Job> jobs(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        const aurora_batch_item& it = items[k];
+        if (it.scene && it.scene->ctx != c) return fail(c, AURORA_ERR_INVALID, "batch item scene belongs to another context");
+        Status s = validate_job(it.scene, &it.camera, &it.render, it.d_rgb_sum, it.d_albedo_sum, it.d_normal_sum,
+                                jobs[k]);
+        if (s.bad()) return fail(c, s.code, "batch item " + std::to_string(k) + ": " + s.msg);
+    }
+    CK(c, render_jobs(c, jobs, stats, false, (flags & AURORA_RENDER_GRAPH) != 0));
     return AURORA_OK;
 }
 

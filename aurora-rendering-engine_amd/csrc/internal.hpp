@@ -16,7 +16,7 @@ constexpr int kPools = 8;
 constexpr uint32_t kPoolStride = 32;
 // render counters: [0], [1] queue counts, pools from kPoolBase
 constexpr uint32_t kPoolBase = 32;
-// 64-bit segment count of the fused tail kernel
+// 64-bit count of ray segments traced in the pass (shade and tail add)
 constexpr uint32_t kTailSegs = kPoolBase + kPools * kPoolStride;
 constexpr size_t kCounterBytes = (kTailSegs + 32) * 4;
 
@@ -83,31 +83,45 @@ struct ExtendArgs {
     uint32_t n;
     uint32_t* pool;           // kPools work counters kPoolStride u32 apart, 0 at launch
     uint32_t* zero_after;     // optional: written 0 by the kernel (next queue count)
-    float tmin, tmax;
+    float tmax;               // ray interval (SceneRec::tmin, tmax)
     float2* hit;              // (t, prim bits)
+    const struct SceneRec* recs;  // scene table; a ray's scene is the bits of ray_d.w
     uint32_t* counts;         // optional: 3 per ray (nodes, spheres, quads fetched)
     unsigned long long* totals; // optional: [nodes, spheres, quads] summed
     uint32_t* spill;          // stack spill, spill_stride * extend_spill_levels() u32
     uint32_t spill_stride;    // threads in the persistent grid
 };
-Status launch_extend(const DevScene& sc, const ExtendArgs& a, int grid, hipStream_t stream);
+Status launch_extend(const ExtendArgs& a, int grid, hipStream_t stream);
 Status launch_brute(const DevScene& sc, const float4* o, const float4* d, uint32_t n, float tmin, float tmax,
                     float2* hit, hipStream_t stream);
 int extend_block();         // threads per block of the extend kernel
 int extend_spill_levels();  // u32 stack entries per thread in ExtendArgs::spill
 
-// Wavefront render.
+// Wavefront render.  A pass renders spp_pass samples of every pixel of one
+// or more scenes; each scene of the pass has a record in a device table and
+// every path carries its scene's index.
 struct CamF {
     v3 center, p00, du, dv, disk_u, disk_v;
     uint32_t W, H;
     uint32_t defocus;
 };
-struct RenderArgs {
+struct SceneRec {
+    DevScene sc;
     CamF cam;
     uint32_t seed, sample_base, spp_pass, max_depth;
     float tmin;
-    v3 bg;
-    bool aov;
+    float bg[3];
+    uint32_t aov;                 // write first-hit albedo / normal
+    uint32_t WH;                  // pixels
+    uint32_t path_off, pix_off;   // first path id / pixel of the scene in the pass
+    float *rgb, *alb, *nrm;       // per-pixel sums (device)
+};
+struct Pass {
+    const SceneRec* recs;  // device table
+    uint32_t n_recs;
+    uint32_t n_paths;      // sum of WH * spp_pass
+    uint32_t n_pixels;     // sum of WH over scenes with spp_pass > 0
+    uint32_t any_aov;
 };
 struct Workspace {
     DevBuf q_o[2], q_d[2], q_b[2];   // path queues (float4 each)
@@ -116,14 +130,15 @@ struct Workspace {
     DevBuf aov_a, aov_n;              // float4 per path id
     DevBuf counters;                  // queue counts [0], [1]; pools from kPoolBase
     DevBuf spill;
+    DevBuf recs;                      // SceneRec table of the current pass
     uint32_t capacity = 0;
-    uint32_t* host_counts = nullptr;  // pinned, 64 slots
+    uint32_t* host_counts = nullptr;  // pinned, 256 slots
 };
-Status launch_raygen(const DevScene& sc, const RenderArgs& r, Workspace& ws, uint32_t n_paths, hipStream_t s);
-Status launch_shade(const DevScene& sc, const RenderArgs& r, Workspace& ws, int cur, uint32_t depth, int grid,
-                    hipStream_t s);
-Status launch_tail(const DevScene& sc, const RenderArgs& r, Workspace& ws, int cur, uint32_t depth, int grid,
-                   hipStream_t s);
-Status launch_accumulate(const RenderArgs& r, Workspace& ws, float* rgb, float* alb, float* nrm, hipStream_t s);
+Status launch_raygen(const Pass& p, Workspace& ws, hipStream_t s);
+Status launch_shade(const Pass& p, Workspace& ws, int cur, uint32_t depth, int grid, hipStream_t s);
+Status launch_tail(const Pass& p, Workspace& ws, int cur, uint32_t depth, int grid, hipStream_t s);
+Status launch_accumulate(const Pass& p, Workspace& ws, hipStream_t s);
+// sample_base += spp_pass for every record (lets a captured pass be replayed)
+Status launch_advance(SceneRec* recs, uint32_t n, hipStream_t s);
 
 }  // namespace aur

@@ -2,10 +2,12 @@
 // compaction) and accumulation.  Extension (closest hit) is k_extend in
 // trace.hip.
 //
-// A pass renders S samples of every pixel as W*H*S paths.  Path ids are
-// sample-major (id = s*W*H + pixel) so the final accumulation reads each
+// A pass renders S samples of every pixel of one or more scenes (a batch):
+// scene k owns path ids [path_off_k, path_off_k + W_k*H_k*S_k), sample-major
+// (id = path_off + s*W*H + pixel), so the final accumulation reads each
 // sample plane coalesced and adds samples in order, exactly as a sequential
-// per-pixel loop would.  Path state lives in HBM as structure-of-arrays
+// per-pixel loop would.  A path's scene index travels in its queue entry
+// (direction.w), and every kernel reads the scene's record from a table.  Path state lives in HBM as structure-of-arrays
 // float4 queues (origin+id, direction, throughput); the only radiance a path
 // ever gains comes at its last segment (a light or the background), so the
 // shade kernel writes it once to L[id] and carries no radiance in the queue.
@@ -16,25 +18,43 @@
 #include "internal.hpp"
 #include "traverse.hpp"
 
+#include <algorithm>
+
 namespace aur {
 
 namespace {
 
 constexpr int kBlock = 256;
 
-__global__ __launch_bounds__(kBlock) void k_raygen(RenderArgs r, uint32_t n, float4* qo, float4* qd, float4* qb,
-                                                   float4* L, uint32_t* counters) {
+// last record whose offset (path_off or pix_off) is <= x
+template <bool PIX>
+__device__ __forceinline__ uint32_t find_rec(const SceneRec* recs, uint32_t n, uint32_t x) {
+    uint32_t lo = 0, hi = n;  // invariant: answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t off = PIX ? recs[mid].pix_off : recs[mid].path_off;
+        if (off <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_raygen(Pass pa, float4* qo, float4* qd, float4* qb, float4* L,
+                                                   uint32_t* counters) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = pa.n_paths;
     if (p == 0) {
         counters[0] = n;   // queue 0 count
         counters[1] = 0u;  // queue 1 count
+        *reinterpret_cast<unsigned long long*>(counters + kTailSegs) = 0ull;
     }
     if (p < kPools) counters[kPoolBase + kPoolStride * p] = 0u;  // extend work pools
-    if (p == 0) *reinterpret_cast<unsigned long long*>(counters + kTailSegs) = 0ull;
     if (p >= n) return;
-    const uint32_t WH = r.cam.W * r.cam.H;
-    const uint32_t s_local = p / WH;
-    const uint32_t pixel = p - s_local * WH;
+    const uint32_t sid = find_rec<false>(pa.recs, pa.n_recs, p);
+    const SceneRec& r = pa.recs[sid];
+    const uint32_t local = p - r.path_off;
+    const uint32_t s_local = local / r.WH;
+    const uint32_t pixel = local - s_local * r.WH;
     const uint32_t i = pixel % r.cam.W, j = pixel / r.cam.W;
     const uint32_t sample = r.sample_base + s_local;
     const u4 q = philox(u4{pixel, sample, 0u, 0u}, r.seed, STREAM_RENDER);
@@ -56,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderArgs r, uint32_t n, flo
     }
     const v3 dir = unitv(ps - org);
     qo[p] = make_float4(org.x, org.y, org.z, __uint_as_float(p));
-    qd[p] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    qd[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(sid));
     qb[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -96,19 +116,22 @@ struct ShadeOut {
 
 // One shading event of path `path` at segment `depth`: background or light
 // emission on termination (written once to L[path]), else scatter.
-__device__ __forceinline__ ShadeOut shade_core(const DevScene& sc, const RenderArgs& r, const ShadeBufs& b,
-                                               v3 org, v3 dir, v3 beta, float t, int prim, uint32_t path,
-                                               uint32_t depth, bool aov, uint32_t WH) {
+__device__ __forceinline__ ShadeOut shade_core(const SceneRec& r, const ShadeBufs& b, v3 org, v3 dir, v3 beta,
+                                               float t, int prim, uint32_t path, uint32_t depth) {
+    const DevScene& sc = r.sc;
+    const bool aov = r.aov && depth == 0;
     ShadeOut out;
     out.alive = false;
-    const uint32_t s_local = path / WH;
-    const uint32_t pixel = path - s_local * WH;
+    const uint32_t local = path - r.path_off;
+    const uint32_t s_local = local / r.WH;
+    const uint32_t pixel = local - s_local * r.WH;
     const uint32_t sample = r.sample_base + s_local;
+    const v3 bg = V(r.bg[0], r.bg[1], r.bg[2]);
     if (prim < 0) {
-        const v3 e = beta * r.bg;
+        const v3 e = beta * bg;
         b.L[path] = make_float4(e.x, e.y, e.z, 0.0f);
         if (aov) {
-            const v3 a = clamp01(r.bg);
+            const v3 a = clamp01(bg);
             b.aov_a[path] = make_float4(a.x, a.y, a.z, 0.0f);
             b.aov_n[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
@@ -192,8 +215,9 @@ __device__ __forceinline__ ShadeOut shade_core(const DevScene& sc, const RenderA
 // Grid-stride over the live queue, one path per thread.  Survivors are
 // appended to the next queue with one atomic per block and iteration: counts
 // are scanned within each wave (ballot popcounts) and across the 4 waves
-// (LDS).
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, RenderArgs r, ShadeBufs b, uint32_t depth) {
+// (LDS).  The segments traced in the pass are counted with one atomic per
+// block.
+__global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t depth) {
     __shared__ uint32_t s_wave[kBlock / 64];
     __shared__ uint32_t s_base;
     const uint32_t n = b.counters[b.cur];
@@ -201,18 +225,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, RenderArgs r, Sha
     if (blockIdx.x == 0 && threadIdx.x < kPools) b.counters[kPoolBase + kPoolStride * threadIdx.x] = 0u;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t WH = r.cam.W * r.cam.H;
-    const bool aov = r.aov && depth == 0;
+    unsigned long long segs = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         const uint32_t i = base + tid;
         ShadeOut o;
         o.alive = false;
-        uint32_t path = 0;
+        uint32_t path = 0, sid = 0;
         if (i < n) {
             const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
             const float2 H = b.hit[i];
             path = __float_as_uint(O.w);
-            o = shade_core(sc, r, b, xyz(O), xyz(D), xyz(B), H.x, __float_as_int(H.y), path, depth, aov, WH);
+            sid = __float_as_uint(D.w);
+            o = shade_core(pa.recs[sid], b, xyz(O), xyz(D), xyz(B), H.x, __float_as_int(H.y), path, depth);
+            ++segs;
         }
         const uint64_t m = __ballot(o.alive);
         if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
@@ -229,35 +254,38 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, RenderArgs r, Sha
         if (o.alive) {
             const uint32_t slot = s_base + before + (uint32_t)__popcll(m & lt);
             b.no[slot] = make_float4(o.p.x, o.p.y, o.p.z, __uint_as_float(path));
-            b.nd[slot] = make_float4(o.nd.x, o.nd.y, o.nd.z, 0.0f);
+            b.nd[slot] = make_float4(o.nd.x, o.nd.y, o.nd.z, __uint_as_float(sid));
             b.nb[slot] = make_float4(o.beta.x, o.beta.y, o.beta.z, 0.0f);
         }
         __syncthreads();  // s_wave / s_base are reused by the next iteration
     }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) segs += __shfl_xor(segs, off);
+    if (lane == 0 && segs) atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kTailSegs), segs);
 }
 
 // Fused tail: once the live queue is short, each thread carries one path
 // from segment `depth` to termination (traverse -> shade -> ...), so the
 // last bounces cost one launch instead of two per bounce.  Same arithmetic
 // as the extend/shade pair, so results are unchanged.
-__global__ __launch_bounds__(kBlock) void k_tail(DevScene sc, RenderArgs r, ShadeBufs b, uint32_t depth0,
-                                                 uint32_t* spill, uint32_t spill_stride) {
+__global__ __launch_bounds__(kBlock) void k_tail(Pass pa, ShadeBufs b, uint32_t depth0, uint32_t* spill,
+                                                 uint32_t spill_stride) {
     __shared__ uint32_t stk[kTravLds * kBlock];
     const uint32_t n = b.counters[b.cur];
-    const uint32_t WH = r.cam.W * r.cam.H;
     uint32_t* my_spill = spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
     unsigned long long segs = 0;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
         const uint32_t path = __float_as_uint(O.w);
+        const SceneRec& r = pa.recs[__float_as_uint(D.w)];
         v3 org = xyz(O), dir = xyz(D), beta = xyz(B);
         for (uint32_t depth = depth0; depth < r.max_depth; ++depth) {
             ++segs;
             float t;
             int prim;
-            traverse_closest(sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill, spill_stride, t,
+            traverse_closest(r.sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill, spill_stride, t,
                              prim);
-            const ShadeOut o = shade_core(sc, r, b, org, dir, beta, t, prim, path, depth, r.aov && depth == 0, WH);
+            const ShadeOut o = shade_core(r, b, org, dir, beta, t, prim, path, depth);
             if (!o.alive) break;
             org = o.p;
             dir = o.nd;
@@ -270,46 +298,46 @@ __global__ __launch_bounds__(kBlock) void k_tail(DevScene sc, RenderArgs r, Shad
         atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kTailSegs), segs);
 }
 
-// per pixel: add the pass's samples in sample order to the caller's sums
-__global__ __launch_bounds__(kBlock) void k_accumulate(uint32_t WH, uint32_t S, const float4* L, const float4* A,
-                                                       const float4* Nn, float* rgb, float* alb, float* nrm) {
-    const uint32_t px = blockIdx.x * kBlock + threadIdx.x;
-    if (px >= WH) return;
+// per pixel: add the pass's samples in sample order to the scene's sums
+__global__ __launch_bounds__(kBlock) void k_accumulate(Pass pa, const float4* L, const float4* A, const float4* Nn) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= pa.n_pixels) return;
+    const SceneRec& r = pa.recs[find_rec<true>(pa.recs, pa.n_recs, q)];
+    const uint32_t px = q - r.pix_off;
+    const uint32_t WH = r.WH, S = r.spp_pass;
+    const float4* Ls = L + r.path_off;
+    float* rgb = r.rgb;
     v3 acc = V(rgb[3 * (size_t)px], rgb[3 * (size_t)px + 1], rgb[3 * (size_t)px + 2]);
-    for (uint32_t s = 0; s < S; ++s) acc = acc + xyz(L[(size_t)s * WH + px]);
+    for (uint32_t s = 0; s < S; ++s) acc = acc + xyz(Ls[(size_t)s * WH + px]);
     rgb[3 * (size_t)px] = acc.x;
     rgb[3 * (size_t)px + 1] = acc.y;
     rgb[3 * (size_t)px + 2] = acc.z;
-    if (alb) {
+    if (r.aov) {
+        const float4* As = A + r.path_off;
+        const float4* Ns = Nn + r.path_off;
+        float* alb = r.alb;
+        float* nrm = r.nrm;
         v3 a = V(alb[3 * (size_t)px], alb[3 * (size_t)px + 1], alb[3 * (size_t)px + 2]);
-        v3 q = V(nrm[3 * (size_t)px], nrm[3 * (size_t)px + 1], nrm[3 * (size_t)px + 2]);
+        v3 c = V(nrm[3 * (size_t)px], nrm[3 * (size_t)px + 1], nrm[3 * (size_t)px + 2]);
         for (uint32_t s = 0; s < S; ++s) {
-            a = a + xyz(A[(size_t)s * WH + px]);
-            q = q + xyz(Nn[(size_t)s * WH + px]);
+            a = a + xyz(As[(size_t)s * WH + px]);
+            c = c + xyz(Ns[(size_t)s * WH + px]);
         }
         alb[3 * (size_t)px] = a.x;
         alb[3 * (size_t)px + 1] = a.y;
         alb[3 * (size_t)px + 2] = a.z;
-        nrm[3 * (size_t)px] = q.x;
-        nrm[3 * (size_t)px + 1] = q.y;
-        nrm[3 * (size_t)px + 2] = q.z;
+        nrm[3 * (size_t)px] = c.x;
+        nrm[3 * (size_t)px + 1] = c.y;
+        nrm[3 * (size_t)px + 2] = c.z;
     }
 }
 
-}  // namespace
-
-Status launch_raygen(const DevScene& sc, const RenderArgs& r, Workspace& ws, uint32_t n, hipStream_t s) {
-    (void)sc;
-    const uint32_t blocks = (n + kBlock - 1) / kBlock;
-    k_raygen<<<blocks ? blocks : 1, kBlock, 0, s>>>(r, n, ws.q_o[0].as<float4>(), ws.q_d[0].as<float4>(),
-                                                    ws.q_b[0].as<float4>(), ws.L.as<float4>(),
-                                                    ws.counters.as<uint32_t>());
-    AUR_HIP(hipGetLastError());
-    return Status::ok();
+__global__ void k_advance(SceneRec* recs, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) recs[k].sample_base += recs[k].spp_pass;
 }
 
-Status launch_shade(const DevScene& sc, const RenderArgs& r, Workspace& ws, int cur, uint32_t depth, int grid,
-                    hipStream_t s) {
+ShadeBufs shade_bufs(Workspace& ws, int cur) {
     ShadeBufs b;
     b.qo = ws.q_o[cur].as<float4>();
     b.qd = ws.q_d[cur].as<float4>();
@@ -323,36 +351,46 @@ Status launch_shade(const DevScene& sc, const RenderArgs& r, Workspace& ws, int 
     b.aov_n = ws.aov_n.as<float4>();
     b.counters = ws.counters.as<uint32_t>();
     b.cur = cur;
-    k_shade<<<grid, kBlock, 0, s>>>(sc, r, b, depth);
+    return b;
+}
+
+}  // namespace
+
+Status launch_raygen(const Pass& p, Workspace& ws, hipStream_t s) {
+    const uint32_t blocks = std::max<uint32_t>(1, (p.n_paths + kBlock - 1) / kBlock);
+    k_raygen<<<blocks, kBlock, 0, s>>>(p, ws.q_o[0].as<float4>(), ws.q_d[0].as<float4>(), ws.q_b[0].as<float4>(),
+                                       ws.L.as<float4>(), ws.counters.as<uint32_t>());
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }
 
-Status launch_tail(const DevScene& sc, const RenderArgs& r, Workspace& ws, int cur, uint32_t depth, int grid,
-                   hipStream_t s) {
-    ShadeBufs b;
-    b.qo = ws.q_o[cur].as<float4>();
-    b.qd = ws.q_d[cur].as<float4>();
-    b.qb = ws.q_b[cur].as<float4>();
+Status launch_shade(const Pass& p, Workspace& ws, int cur, uint32_t depth, int grid, hipStream_t s) {
+    k_shade<<<grid, kBlock, 0, s>>>(p, shade_bufs(ws, cur), depth);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status launch_tail(const Pass& p, Workspace& ws, int cur, uint32_t depth, int grid, hipStream_t s) {
+    ShadeBufs b = shade_bufs(ws, cur);
     b.hit = nullptr;
     b.no = b.nd = b.nb = nullptr;
-    b.L = ws.L.as<float4>();
-    b.aov_a = ws.aov_a.as<float4>();
-    b.aov_n = ws.aov_n.as<float4>();
-    b.counters = ws.counters.as<uint32_t>();
-    b.cur = cur;
     const size_t need = (size_t)grid * kBlock * (kTravMaxStack - kTravLds) * 4;
     if (need > ws.spill.bytes) return Status{AURORA_ERR_RUNTIME, "tail grid exceeds the stack spill buffer"};
-    k_tail<<<grid, kBlock, 0, s>>>(sc, r, b, depth, ws.spill.as<uint32_t>(), (uint32_t)(grid * kBlock));
+    k_tail<<<grid, kBlock, 0, s>>>(p, b, depth, ws.spill.as<uint32_t>(), (uint32_t)(grid * kBlock));
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }
 
-Status launch_accumulate(const RenderArgs& r, Workspace& ws, float* rgb, float* alb, float* nrm, hipStream_t s) {
-    const uint32_t WH = r.cam.W * r.cam.H;
-    k_accumulate<<<(WH + kBlock - 1) / kBlock, kBlock, 0, s>>>(WH, r.spp_pass, ws.L.as<float4>(),
-                                                               ws.aov_a.as<float4>(), ws.aov_n.as<float4>(), rgb,
-                                                               r.aov ? alb : nullptr, r.aov ? nrm : nullptr);
+Status launch_accumulate(const Pass& p, Workspace& ws, hipStream_t s) {
+    if (p.n_pixels == 0) return Status::ok();
+    k_accumulate<<<(p.n_pixels + kBlock - 1) / kBlock, kBlock, 0, s>>>(p, ws.L.as<float4>(), ws.aov_a.as<float4>(),
+                                                                       ws.aov_n.as<float4>());
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status launch_advance(SceneRec* recs, uint32_t n, hipStream_t s) {
+    k_advance<<<(n + 63) / 64, 64, 0, s>>>(recs, n);
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }

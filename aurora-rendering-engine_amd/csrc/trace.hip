@@ -30,7 +30,7 @@ constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kNoPrim = 0x7fffffffu;
 
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, ExtendArgs a) {
+__global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
     __shared__ uint32_t stk[kLds][kBlock];
     const uint32_t n = a.count ? *a.count : a.n;
     if (a.zero_after && blockIdx.x == 0 && threadIdx.x == 0) *a.zero_after = 0u;
@@ -39,8 +39,13 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, ExtendArgs a) {
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t* spill = a.spill + (size_t)blockIdx.x * kBlock + tid;
     const size_t sstride = a.spill_stride;
-    const float4* N = sc.nodes;
-    const float tmin = a.tmin, tmax = a.tmax;
+    const float tmax = a.tmax;
+    float tmin = 0.0f;  // the scene's ray offset, loaded with the ray
+    // the scene of the lane's current ray (loaded when the ray is claimed)
+    const float4* N = nullptr;
+    const float4* S = nullptr;
+    const float4* Qd = nullptr;
+    uint32_t n_spheres = 0, n_nodes = 0, n_prims = 0;
 
     uint32_t ri = kNone;
     bool done = false;
@@ -52,17 +57,16 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, ExtendArgs a) {
     uint32_t cn = 0, cs = 0, cq = 0;                  // per-ray counts
     unsigned long long tn = 0, ts = 0, tq = 0;        // per-lane totals
 
-    const uint32_t n_prims = sc.n_spheres + sc.n_quads;
     auto test_prim = [&](uint32_t id) {
         float t;
         bool h;
         if (id >= n_prims) return;
-        if (id < sc.n_spheres) {
+        if (id < n_spheres) {
             if (COUNT) ++cs;
-            h = hit_sphere(sc.spheres[id], o, d, tmin, tmax, t);
+            h = hit_sphere(S[id], o, d, tmin, tmax, t);
         } else {
             if (COUNT) ++cq;
-            const float4* q = sc.quads + 3 * (size_t)(id - sc.n_spheres);
+            const float4* q = Qd + 3 * (size_t)(id - n_spheres);
             float al, be;
             h = hit_quad(q[0], q[1], q[2], o, d, tmin, tmax, t, al, be);
         }
@@ -125,12 +129,21 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, ExtendArgs a) {
                     const float4 ro = a.ray_o[ri], rd = a.ray_d[ri];
                     o = xyz(ro);
                     d = xyz(rd);
+                    const SceneRec& rec = a.recs[__float_as_uint(rd.w)];
+                    const DevScene& sc = rec.sc;
+                    tmin = rec.tmin;
+                    N = sc.nodes;
+                    S = sc.spheres;
+                    Qd = sc.quads;
+                    n_spheres = sc.n_spheres;
+                    n_nodes = sc.n_nodes;
+                    n_prims = sc.n_spheres + sc.n_quads;
                     inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
                     best_t = tmax;
                     best_id = kNoPrim;
                     sp = 0;
                     cn = cs = cq = 0;
-                    if (sc.n_nodes == 0) {
+                    if (n_nodes == 0) {
                         finish();
                     } else {
                         const float4 r0 = N[0], r1 = N[1];
@@ -155,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, ExtendArgs a) {
 
         // a well-formed tree only holds even pair indices below n_nodes; any
         // other value ends the ray instead of reading out of bounds
-        if ((cur & 1u) || cur + 1 >= sc.n_nodes) {
+        if ((cur & 1u) || cur + 1 >= n_nodes) {
             finish();
             continue;
         }
@@ -242,10 +255,10 @@ __global__ __launch_bounds__(kBlock) void k_brute(DevScene sc, const float4* ro,
 int extend_block() { return kBlock; }
 int extend_spill_levels() { return kMaxStack - kLds; }
 
-Status launch_extend(const DevScene& sc, const ExtendArgs& a, int grid, hipStream_t s) {
+Status launch_extend(const ExtendArgs& a, int grid, hipStream_t s) {
     if ((size_t)grid * kBlock > a.spill_stride) return Status{AURORA_ERR_RUNTIME, "extend grid exceeds spill stride"};
-    if (a.counts || a.totals) k_extend<true><<<grid, kBlock, 0, s>>>(sc, a);
-    else k_extend<false><<<grid, kBlock, 0, s>>>(sc, a);
+    if (a.counts || a.totals) k_extend<true><<<grid, kBlock, 0, s>>>(a);
+    else k_extend<false><<<grid, kBlock, 0, s>>>(a);
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }

@@ -95,6 +95,11 @@ def stress_section(ctx, n_rays=1 << 22, render_spp=1):
 WEIGHTS = os.path.join(ROOT, "aurora-rendering-engine_amd", "weights", "unet_cornell.f32")
 
 
+def note(msg):
+    """Progress line on stderr (the JSON result alone goes to stdout)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=4096):
     """Time to reach relMSE <= target on the Cornell box at the bench
     resolution, plain progressive rendering versus render + denoise.  The
@@ -119,6 +124,7 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=4096):
         return time.perf_counter() - t0
 
     ra, rb = buf(), buf()
+    note(f"time-to-relMSE: reference 2 x {ref_spp} spp at {W}x{H}")
     render_into(ra, ref_spp, 1 << 28)
     render_into(rb, ref_spp, (1 << 28) + ref_spp)
     A, B = ra / ref_spp, rb / ref_spp
@@ -132,6 +138,7 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=4096):
         done = cp
         err = unbiased_relmse(acc / done, A, B)
         curve.append([done, err])
+        note(f"plain {done} spp: relMSE {err:.3e} ({t_plain:.2f} s)")
         if err <= target:
             break
     out["plain"] = {"spp": done, "seconds": t_plain, "relmse": err, "reached": err <= target, "curve": curve}
@@ -150,6 +157,7 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=4096):
             e = unbiased_relmse(img, A, B)
             e_plain = unbiased_relmse(rgb / spp, A, B)
             dcurve.append([spp, e, e_plain, tr, td])
+            note(f"denoised {spp} spp: relMSE {e:.3e} (plain {e_plain:.3e}; {tr + td:.3f} s)")
             if e <= target:
                 best = {"spp": spp, "seconds": tr + td, "render_s": tr, "denoise_s": td, "relmse": e}
                 break
@@ -160,6 +168,66 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=4096):
     else:
         out["denoised"] = None
     return out
+
+
+def serving_section(ctx, n_scenes=64, seed0=0):
+    """Batched serving: n_scenes seeded scenes of uneven size (10..200k
+    primitives, 256^2..1024^2, 1..16 spp) rendered in shared passes with an
+    autotuned launch shape; aggregate paths/s and per-image latency (batch
+    completion plus that image's denoise, images denoised in order)."""
+    import numpy as np
+    import torch
+
+    from aurora_amd import autotune, scenes
+    from aurora_amd.core import RenderSettings, render_batch
+
+    t0 = time.perf_counter()
+    items = []
+    for s in range(seed0, seed0 + n_scenes):
+        arr, cam, spp = scenes.batch_scene(s)
+        sc = ctx.upload(arr)
+        sc.build_bvh()
+        items.append((sc, cam, RenderSettings(spp=spp, max_depth=16, background=(0.6, 0.7, 0.9), aov=True)))
+    setup = time.perf_counter() - t0
+    bufs = [tuple(torch.zeros((c.image_height, c.image_width, 3), dtype=torch.float32, device="cuda")
+                  for _ in range(3)) for _, c, _ in items]
+
+    def run(graph):
+        for b in bufs:
+            for t in b:
+                t.zero_()
+        return render_batch(ctx, [(sc, cam, rs, *b) for (sc, cam, rs), b in zip(items, bufs)], graph=graph)
+
+    note(f"serving: {n_scenes} scenes uploaded ({setup:.1f} s); autotuning")
+    best, table = autotune.tune(ctx, run, autotune.signature(items), cache_path=None, log=note,
+                                candidates={"extend_blocks_per_cu": [4, 8, 16], "shade_blocks_per_cu": [8],
+                                            "tail_paths": [1 << 18, 1 << 20], "graph": [False, True]})
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = run(best["graph"])
+    t_batch = time.perf_counter() - t1
+    lat = []
+    den = None
+    if os.path.exists(WEIGHTS):
+        from aurora_amd.denoiser import Denoiser
+        den = Denoiser(ctx)
+        den.load(WEIGHTS)
+    t_done = t_batch
+    for (sc, cam, rs), (rgb, alb, nrm) in zip(items, bufs):
+        if den is not None:
+            t2 = time.perf_counter()
+            den.denoise(rgb, alb, nrm, rs.spp)
+            t_done += time.perf_counter() - t2
+        lat.append(t_done)
+    for sc, _, _ in items:
+        sc.close()
+    paths = sum(c.image_width * c.image_height * rs.spp for _, c, rs in items)
+    prims = [sc.arrays.n_prims for sc, _, _ in items]
+    return {"scenes": n_scenes, "prims_min": int(min(prims)), "prims_max": int(max(prims)), "paths": paths,
+            "batch_s": t_batch, "paths_per_s": paths / t_batch, "segments": st["segments"],
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "denoised": den is not None, "autotuned": {k: best[k] for k in best if k != "seconds"},
+            "autotune_table": table, "setup_s": setup}
 
 
 def main():
@@ -174,7 +242,8 @@ def main():
     ap.add_argument("--sweep-tail", action="store_true", help="time several fused-tail switch points")
     ap.add_argument("--no-stress", action="store_true", help="skip the traversal stress scene")
     ap.add_argument("--no-metric", action="store_true", help="skip time-to-relMSE (plain vs denoised)")
-    ap.add_argument("--only", choices=["stress", "copy"], default=None,
+    ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving section")
+    ap.add_argument("--only", choices=["stress", "copy", "serving"], default=None,
                     help="profiling modes: the stress section alone, or a plain 1 GiB device copy "
                          "(calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -200,6 +269,9 @@ def main():
     ctx = aurora_amd.Context(0)
     if args.only == "stress":
         print(json.dumps(stress_section(ctx)), flush=True)
+        return
+    if args.only == "serving":
+        print(json.dumps(serving_section(ctx)), flush=True)
         return
     if args.tail_paths is not None:
         ctx.set_launch(tail_paths=args.tail_paths)
@@ -281,6 +353,10 @@ def main():
     if not args.no_stress:
         scene.close()
         out["stress"] = stress_section(ctx)
+    if not args.no_serving:
+        ctx.set_launch()
+        out["serving"] = serving_section(ctx)
+        ctx.set_launch()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(arr, cam, S)
     print(json.dumps(out), flush=True)

@@ -148,6 +148,9 @@ typedef struct aurora_camera_desc {
 #define AURORA_RENDER_COUNT_FETCHES 2u  /* count BVH node and primitive
                                            records the traversal fetches
                                            (slower; for roofline accounting) */
+#define AURORA_RENDER_GRAPH 4u  /* capture one pass in a hipGraph and replay
+                                   it for every pass of the same shape
+                                   (fixed launch shapes; same results) */
 
 typedef struct aurora_render_desc {
     uint32_t spp;            /* samples this call adds                       */
@@ -272,6 +275,25 @@ int aurora_render_device(aurora_scene* scene, const aurora_camera_desc* cam,
                          const aurora_render_desc* rd, float* d_rgb_sum,
                          float* d_albedo_sum, float* d_normal_sum,
                          aurora_render_stats* stats);
+
+/* One image of a batched render. */
+typedef struct aurora_batch_item {
+    aurora_scene* scene;
+    aurora_camera_desc camera;
+    aurora_render_desc render;
+    float* d_rgb_sum;        /* device, W*H*3                                */
+    float* d_albedo_sum;     /* device, with AURORA_RENDER_AOV               */
+    float* d_normal_sum;
+} aurora_batch_item;
+/* Render many scenes (uneven sizes, resolutions and sample counts) in
+ * shared wavefront passes: every launch of a pass (raygen, traversal,
+ * shading, fused tail, accumulation) serves all of them, each path carrying
+ * its scene's index.  Every image equals its own aurora_render_device
+ * result bit for bit.  flags: AURORA_RENDER_GRAPH.  No reference
+ * counterpart (ARE renders one fixed scene per program). */
+int aurora_render_batch_device(aurora_ctx* ctx, const aurora_batch_item* items,
+                               uint32_t n, uint32_t flags,
+                               aurora_render_stats* stats);
 
 /* Resolve a radiance sum to 8-bit sRGB-ish bytes: x = clamp(sqrt(sum/spp),
  * 0, 0.999) * 256, truncated; byte layout of a P6 PPM body

@@ -1,0 +1,66 @@
+"""Autotuner logic on the CPU: a fake context records the launch shapes,
+a fake workload's cost depends on the shape; the tuner must pick the
+cheapest, leave it applied, and reuse the cached choice without timing."""
+import json
+
+from aurora_amd import autotune
+
+
+class FakeCtx:
+    def __init__(self):
+        self.shape = None
+
+    def set_launch(self, e, s, t):
+        self.shape = (e, s, t)
+
+
+def test_picks_cheapest_and_caches(tmp_path, monkeypatch):
+    ctx = FakeCtx()
+    calls = []
+    clock = [0.0]
+
+    def cost(shape, graph):
+        e, s, t = shape
+        return 1.0 + abs(e - 8) + (0.5 if t != 1 << 20 else 0.0) + (0.0 if graph else 0.25)
+
+    def run(graph):
+        calls.append((ctx.shape, graph))
+        clock[0] += cost(ctx.shape, graph)
+
+    monkeypatch.setattr(autotune.time, "perf_counter", lambda: clock[0])
+    cache = tmp_path / "tune.json"
+    best, table = autotune.tune(ctx, run, "sigA", cache_path=str(cache))
+    assert (best["extend_blocks_per_cu"], best["tail_paths"], best["graph"]) == (8, 1 << 20, True)
+    assert ctx.shape == (8, 8, 1 << 20)
+    n_cand = 3 * 1 * 3 * 2
+    assert len(table) == n_cand and len(calls) == n_cand * 3  # warm-up + 2 repeats each
+    saved = json.loads(cache.read_text())
+    assert saved["sigA"]["best"]["graph"] is True
+
+    ctx2 = FakeCtx()
+    calls.clear()
+    best2, _ = autotune.tune(ctx2, run, "sigA", cache_path=str(cache))
+    assert not calls and ctx2.shape == (8, 8, 1 << 20) and best2 == best
+
+
+def test_signature_separates_workloads():
+    class Cam:
+        def __init__(self, w):
+            self.image_width = self.image_height = w
+
+    class RS:
+        def __init__(self, spp):
+            self.spp = spp
+
+    class Arr:
+        def __init__(self, n):
+            self.n_prims = n
+
+    class Sc:
+        def __init__(self, n):
+            self.arrays = Arr(n)
+
+    a = autotune.signature([(Sc(10), Cam(64), RS(4))])
+    b = autotune.signature([(Sc(10), Cam(64), RS(8))])
+    c = autotune.signature([(Sc(11), Cam(64), RS(4))])
+    assert len({a, b, c}) == 3
