@@ -28,6 +28,8 @@ TEX_IMAGE = 3
 RENDER_AOV = 1
 RENDER_COUNT_FETCHES = 2
 RENDER_GRAPH = 4
+LOSS_LOG_L2 = 0
+LOSS_REL_L2 = 1
 
 NODE_LEAF = 0x80000000
 MISS = -1
@@ -154,6 +156,7 @@ EXPORTS = [
     ("aurora_synchronize", C.c_int, [_vp]),
     ("aurora_stream", _vp, [_vp]),
     ("aurora_set_launch", C.c_int, [_vp, C.c_int, C.c_int, _u32]),
+    ("aurora_set_ray_sort", C.c_int, [_vp, C.c_int]),
     ("aurora_scene_upload", C.c_int, [_vp, C.POINTER(SceneDesc), C.POINTER(_vp)]),
     ("aurora_scene_destroy", None, [_vp]),
     ("aurora_bvh_build", C.c_int, [_vp, C.POINTER(BvhStats)]),
@@ -176,6 +179,7 @@ EXPORTS = [
     ("aurora_denoiser_get_params", C.c_int, [_vp, _vp, C.c_uint64]),
     ("aurora_denoiser_set_params", C.c_int, [_vp, _vp, C.c_uint64]),
     ("aurora_denoise", C.c_int, [_vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp]),
+    ("aurora_denoiser_set_loss", C.c_int, [_vp, C.c_int]),
     ("aurora_denoiser_loss_grad", C.c_int,
      [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _u32, C.POINTER(C.c_float), _vp]),
     ("aurora_denoiser_train_step", C.c_int,

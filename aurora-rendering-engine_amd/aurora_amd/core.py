@@ -175,6 +175,11 @@ class Context:
     def set_launch(self, extend_blocks_per_cu=8, shade_blocks_per_cu=8, tail_paths=1 << 20):
         self.check(self.lib.aurora_set_launch(self.h, extend_blocks_per_cu, shade_blocks_per_cu, tail_paths))
 
+    def set_ray_sort(self, mode=1):
+        """Ray queries: 0 trace in the given order, 1 sort large queries on
+        large trees (default), 2 always sort.  Hits are the same."""
+        self.check(self.lib.aurora_set_ray_sort(self.h, mode))
+
     def synchronize(self):
         self.check(self.lib.aurora_synchronize(self.h))
 

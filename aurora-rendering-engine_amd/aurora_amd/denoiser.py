@@ -75,6 +75,12 @@ class Denoiser:
         self.ctx.synchronize()
         return out
 
+    def set_loss(self, kind):
+        """"log" (L2 in log(1 + radiance)) or "rel" (relative L2, the error
+        metric's form)."""
+        code = {"log": abi.LOSS_LOG_L2, "rel": abi.LOSS_REL_L2}[kind]
+        self.ctx.check(self.lib.aurora_denoiser_set_loss(self.h, code))
+
     def loss_grad(self, rgb, alb, nrm, target, spp, grads=True):
         B, H, W = rgb.shape[0], rgb.shape[1], rgb.shape[2]
         loss = C.c_float(0)

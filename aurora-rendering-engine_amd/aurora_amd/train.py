@@ -47,7 +47,7 @@ def make_dataset(ctx, seeds, size, in_spp, target_spp, progress=True):
     for k in in_spp:
         data[k] = [torch.zeros((n, size, size, 3), dtype=torch.float32, device="cuda") for _ in range(3)]
     cam = scenes.cornell_camera(size)
-    t0 = time.time()
+    t0 = t_log = time.time()
     for i, sd in enumerate(seeds):
         scene = ctx.upload(scenes.cornell_variant(sd))
         scene.build_bvh()
@@ -59,8 +59,9 @@ def make_dataset(ctx, seeds, size, in_spp, target_spp, progress=True):
             off += k
             data[k][0][i], data[k][1][i], data[k][2][i] = r, a, nm
         scene.close()
-        if progress and (i % 16 == 0 or i == n - 1):
-            log(f"dataset: scene {i + 1}/{n} rendered ({time.time() - t0:.1f} s)")
+        if progress and (i % 16 == 0 or i == n - 1 or time.time() - t_log > 30):
+            t_log = time.time()
+            log(f"dataset: scene {i + 1}/{n} at {size}x{size} rendered ({t_log - t0:.1f} s)")
     return data
 
 
@@ -140,6 +141,8 @@ def main():
     ap.add_argument("--hires-in-spp", type=str, default="256,1024")
     ap.add_argument("--hires-target-spp", type=int, default=2048)
     ap.add_argument("--init", type=str, default=None, help="start from these weights")
+    ap.add_argument("--loss", choices=["rel", "log"], default="rel",
+                    help="relative L2 (the metric's form) or L2 in log(1 + radiance)")
     ap.add_argument("--out", type=str, required=True)
     ap.add_argument("--report", type=str, default=None)
     args = ap.parse_args()
@@ -157,6 +160,7 @@ def main():
                                  args.hires_target_spp))
     t_data = time.time() - t0
     net = Denoiser(ctx, seed=0)
+    net.set_loss(args.loss)
     if args.init:
         net.load(args.init)
     before = evaluate(net, held)
@@ -167,7 +171,7 @@ def main():
     after = evaluate(net, held)
     rep = {"scenes": args.scenes, "size": args.size, "in_spp": in_spp, "target_spp": args.target_spp,
            "hires_scenes": args.hires_scenes, "hires_size": args.hires_size,
-           "steps": args.steps, "batch": args.batch, "crop": args.crop, "dataset_s": t_data, "train_s": t_train,
+           "steps": args.steps, "batch": args.batch, "loss": args.loss, "init": args.init, "crop": args.crop, "dataset_s": t_data, "train_s": t_train,
            "final_loss": float(np.mean(hist[-100:])), "heldout_before": before, "heldout_after": after}
     log(json.dumps(rep))
     if args.report:

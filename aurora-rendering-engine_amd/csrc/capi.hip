@@ -20,6 +20,11 @@
 
 using namespace aur;
 
+// automatic ray sorting of ray queries: at least this many rays on a tree of
+// at least this many nodes (32 bytes each: 64 MiB, beyond what the L2s hold)
+constexpr uint32_t kSortMinRays = 1u << 20;
+constexpr uint32_t kSortMinNodes = 1u << 21;
+
 struct aurora_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -29,6 +34,7 @@ struct aurora_ctx {
     int shade_grid = 0;
     uint32_t tail_paths = 1u << 20;  // fused-tail switch: live paths at or below
     uint32_t graph_tail_depth = 8;   // captured passes switch to the fused tail here
+    int ray_sort = 1;                // ray queries: 0 never sort, 1 large scenes, 2 always
     Workspace ws;                    // path queues shared by the context's renders
     SceneRec* host_recs = nullptr;   // pinned staging of the pass's scene table
     uint32_t host_recs_cap = 0;
@@ -45,6 +51,7 @@ struct aurora_scene {
     aurora_bvh_stats stats{};
     DevBuf rec1;  // one-entry scene table for ray queries
     DevBuf tr_o, tr_d, tr_hit, tr_spill, tr_pool, totals;
+    DevBuf tr_key[2], tr_val[2], tr_hist;  // ray sort of large queries
 };
 
 namespace {
@@ -279,14 +286,38 @@ Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n,
         }
         a.spill = s->tr_spill.as<uint32_t>();
         a.spill_stride = (uint32_t)(grid * extend_block());
+        a.recs = s->rec1.as<SceneRec>();
+        a.one_scene = 1;
+        // Large queries on large trees: claim rays in (origin Morton code,
+        // octant) order, so the rays in flight on one XCD share the nodes
+        // its L2 holds.  The hit of each ray is unchanged.
+        const bool sort = s->stats.n_nodes > 0 &&
+                          (c->ray_sort == 2 ||
+                           (c->ray_sort == 1 && n >= kSortMinRays && s->stats.n_nodes >= kSortMinNodes));
+        if (sort) {
+            for (int k = 0; k < 2; ++k) {
+                AUR_TRY(s->tr_key[k].alloc((size_t)n * 4));
+                AUR_TRY(s->tr_val[k].alloc((size_t)n * 4));
+            }
+            AUR_TRY(s->tr_hist.alloc(radix_hist_bytes(n)));
+            a.order = s->tr_val[0].as<uint32_t>();
+        }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (ts) {
             AUR_HIP(hipEventCreate(&e0));
             AUR_HIP(hipEventCreate(&e1));
             AUR_HIP(hipEventRecord(e0, c->stream));
         }
-        a.recs = s->rec1.as<SceneRec>();
-        Status ls = launch_extend(a, grid, c->stream);
+        Status ls;
+        if (sort) {
+            ls = launch_ray_keys(a.ray_o, a.ray_d, n, s->view.nodes, s->tr_key[0].as<uint32_t>(),
+                                 s->tr_val[0].as<uint32_t>(), c->stream);
+            if (!ls.bad())
+                ls = radix_sort_pairs(s->tr_key[0].as<uint32_t>(), s->tr_val[0].as<uint32_t>(),
+                                      s->tr_key[1].as<uint32_t>(), s->tr_val[1].as<uint32_t>(),
+                                      s->tr_hist.as<uint32_t>(), n, 4, c->stream);
+        }
+        if (!ls.bad()) ls = launch_extend(a, grid, c->stream);
         if (ts) {
             Status es = hip_status(hipEventRecord(e1, c->stream), "hipEventRecord");
             if (!es.bad()) es = hip_status(hipEventSynchronize(e1), "hipEventSynchronize");
@@ -495,6 +526,7 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         a.tmax = INFINITY;
         a.hit = w.hit.as<float2>();
         a.recs = w.recs.as<SceneRec>();
+        a.one_scene = jobs.size() == 1;
         a.totals = count_fetches ? jobs[0].scene->totals.as<unsigned long long>() : nullptr;
         a.spill = w.spill.as<uint32_t>();
         a.spill_stride = (uint32_t)(std::max(ext_grid, graph ? graph_tail_grid : tail_grid) * extend_block());
@@ -700,6 +732,13 @@ int aurora_set_launch(aurora_ctx* c, int ext_blocks_per_cu, int shade_blocks_per
     return AURORA_OK;
 }
 
+int aurora_set_ray_sort(aurora_ctx* c, int mode) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (mode < 0 || mode > 2) return fail(c, AURORA_ERR_INVALID, "ray sort mode must be 0, 1 or 2");
+    c->ray_sort = mode;
+    return AURORA_OK;
+}
+
 int aurora_scene_upload(aurora_ctx* c, const aurora_scene_desc* d, aurora_scene** out) {
     if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
     if (!d || !out) return fail(c, AURORA_ERR_INVALID, "desc or out is NULL");
@@ -774,7 +813,8 @@ void aurora_scene_destroy(aurora_scene* s) {
                       &s->scr.bounds, &s->scr.keys[0], &s->scr.keys[1], &s->scr.vals[0], &s->scr.vals[1],
                       &s->scr.hist, &s->scr.gamma, &s->scr.flags, &s->scr.int_parent, &s->scr.leaf_parent,
                       &s->scr.int_pos, &s->scr.leaf_pos, &s->scr.arrive, &s->scr.depth, &s->rec1, &s->tr_o, &s->tr_d,
-                      &s->tr_hit, &s->tr_spill, &s->tr_pool, &s->totals};
+                      &s->tr_hit, &s->tr_spill, &s->tr_pool, &s->totals, &s->tr_key[0], &s->tr_key[1],
+                      &s->tr_val[0], &s->tr_val[1], &s->tr_hist};
     for (DevBuf* b : bufs) b->release();
     delete s;
 }

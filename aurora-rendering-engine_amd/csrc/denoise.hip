@@ -86,6 +86,14 @@ int aurora_denoise(aurora_denoiser* d, const float* rgb, const float* alb, const
     return AURORA_OK;
 }
 
+int aurora_denoiser_set_loss(aurora_denoiser* d, int kind) {
+    if (!d) return ctx_fail(nullptr, AURORA_ERR_INVALID, "denoiser is NULL");
+    if (kind != AURORA_LOSS_LOG_L2 && kind != AURORA_LOSS_REL_L2)
+        return ctx_fail(d->ctx, AURORA_ERR_INVALID, "loss kind must be AURORA_LOSS_LOG_L2 or AURORA_LOSS_REL_L2");
+    d->net->set_loss(kind);
+    return AURORA_OK;
+}
+
 int aurora_denoiser_loss_grad(aurora_denoiser* d, const float* rgb, const float* alb, const float* nrm,
                               const float* target, uint32_t batch, uint32_t spp, uint32_t width, uint32_t height,
                               float* loss, float* host_grads) {

@@ -75,6 +75,13 @@ struct BvhScratch {
 Status lbvh_build(const DevScene& sc, uint32_t n_prims, DevBuf& nodes, DevBuf& morton, DevBuf& order,
                   BvhScratch& scr, hipStream_t stream, aurora_bvh_stats* stats);
 
+// Stable LSD radix sort of (key, value) pairs, 8 bits per pass from bit 0.
+// Sorted pairs end in (k0, v0) for an even number of passes, else in
+// (k1, v1).  hist holds radix_hist_bytes(n).
+size_t radix_hist_bytes(uint32_t n);
+Status radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, uint32_t n,
+                        int passes, hipStream_t stream);
+
 // Extend (closest hit) launch parameters.
 struct ExtendArgs {
     const float4* ray_o;      // (origin, any)
@@ -86,6 +93,8 @@ struct ExtendArgs {
     float tmax;               // ray interval (SceneRec::tmin, tmax)
     float2* hit;              // (t, prim bits)
     const struct SceneRec* recs;  // scene table; a ray's scene is the bits of ray_d.w
+    uint32_t one_scene;       // nonzero: every ray belongs to recs[0]
+    const uint32_t* order;    // optional: claim order of the rays (slot -> ray index)
     uint32_t* counts;         // optional: 3 per ray (nodes, spheres, quads fetched)
     unsigned long long* totals; // optional: [nodes, spheres, quads] summed
     uint32_t* spill;          // stack spill, spill_stride * extend_spill_levels() u32
@@ -94,6 +103,11 @@ struct ExtendArgs {
 Status launch_extend(const ExtendArgs& a, int grid, hipStream_t stream);
 Status launch_brute(const DevScene& sc, const float4* o, const float4* d, uint32_t n, float tmin, float tmax,
                     float2* hit, hipStream_t stream);
+// Sort keys of n rays for traversal order: 27-bit Morton code of the origin
+// within the box of the scene's root node (origins outside are clamped),
+// then the 3-bit direction octant; values are the ray indices.
+Status launch_ray_keys(const float4* ray_o, const float4* ray_d, uint32_t n, const float4* nodes, uint32_t* keys,
+                       uint32_t* vals, hipStream_t stream);
 int extend_block();         // threads per block of the extend kernel
 int extend_spill_levels();  // u32 stack entries per thread in ExtendArgs::spill
 

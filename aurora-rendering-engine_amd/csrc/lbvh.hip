@@ -342,6 +342,24 @@ inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + pe
 
 }  // namespace
 
+size_t radix_hist_bytes(uint32_t n) { return (size_t)256 * blocks_for(n, kTile) * 4; }
+
+Status radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, uint32_t n,
+                        int passes, hipStream_t st) {
+    if (n == 0) return Status::ok();
+    const uint32_t nb = blocks_for(n, kTile);
+    uint32_t* ks[2] = {k0, k1};
+    uint32_t* vs[2] = {v0, v1};
+    for (int pass = 0; pass < passes; ++pass) {
+        const int a = pass & 1, b = a ^ 1;
+        k_hist<<<nb, kBlock, 0, st>>>(ks[a], n, 8 * pass, hist, nb);
+        k_scan<<<1, 1024, 0, st>>>(hist, 256u * nb);
+        k_scatter<<<nb, kBlock, 0, st>>>(ks[a], vs[a], ks[b], vs[b], n, 8 * pass, hist, nb);
+    }
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
 Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& morton, DevBuf& order,
                   BvhScratch& s, hipStream_t st, aurora_bvh_stats* stats) {
     DevScene sc = sc_in;

@@ -39,6 +39,9 @@ __device__ __forceinline__ uint32_t find_rec(const SceneRec* recs, uint32_t n, u
     return lo;
 }
 
+// ONE (in every kernel below): the pass holds a single scene, so its record
+// is read at a wave-uniform address (scalar loads) instead of per path.
+template <bool ONE>
 __global__ __launch_bounds__(kBlock) void k_raygen(Pass pa, float4* qo, float4* qd, float4* qb, float4* L,
                                                    uint32_t* counters) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
@@ -50,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Pass pa, float4* qo, float4* 
     }
     if (p < kPools) counters[kPoolBase + kPoolStride * p] = 0u;  // extend work pools
     if (p >= n) return;
-    const uint32_t sid = find_rec<false>(pa.recs, pa.n_recs, p);
+    const uint32_t sid = ONE ? 0u : find_rec<false>(pa.recs, pa.n_recs, p);
     const SceneRec& r = pa.recs[sid];
     const uint32_t local = p - r.path_off;
     const uint32_t s_local = local / r.WH;
@@ -217,6 +220,7 @@ __device__ __forceinline__ ShadeOut shade_core(const SceneRec& r, const ShadeBuf
 // are scanned within each wave (ballot popcounts) and across the 4 waves
 // (LDS).  The segments traced in the pass are counted with one atomic per
 // block.
+template <bool ONE>
 __global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t depth) {
     __shared__ uint32_t s_wave[kBlock / 64];
     __shared__ uint32_t s_base;
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t
             const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
             const float2 H = b.hit[i];
             path = __float_as_uint(O.w);
-            sid = __float_as_uint(D.w);
+            sid = ONE ? 0u : __float_as_uint(D.w);
             o = shade_core(pa.recs[sid], b, xyz(O), xyz(D), xyz(B), H.x, __float_as_int(H.y), path, depth);
             ++segs;
         }
@@ -268,6 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t
 // from segment `depth` to termination (traverse -> shade -> ...), so the
 // last bounces cost one launch instead of two per bounce.  Same arithmetic
 // as the extend/shade pair, so results are unchanged.
+template <bool ONE>
 __global__ __launch_bounds__(kBlock) void k_tail(Pass pa, ShadeBufs b, uint32_t depth0, uint32_t* spill,
                                                  uint32_t spill_stride) {
     __shared__ uint32_t stk[kTravLds * kBlock];
@@ -277,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(Pass pa, ShadeBufs b, uint32_t 
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
         const uint32_t path = __float_as_uint(O.w);
-        const SceneRec& r = pa.recs[__float_as_uint(D.w)];
+        const SceneRec& r = pa.recs[ONE ? 0u : __float_as_uint(D.w)];
         v3 org = xyz(O), dir = xyz(D), beta = xyz(B);
         for (uint32_t depth = depth0; depth < r.max_depth; ++depth) {
             ++segs;
@@ -299,10 +304,11 @@ __global__ __launch_bounds__(kBlock) void k_tail(Pass pa, ShadeBufs b, uint32_t 
 }
 
 // per pixel: add the pass's samples in sample order to the scene's sums
+template <bool ONE>
 __global__ __launch_bounds__(kBlock) void k_accumulate(Pass pa, const float4* L, const float4* A, const float4* Nn) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= pa.n_pixels) return;
-    const SceneRec& r = pa.recs[find_rec<true>(pa.recs, pa.n_recs, q)];
+    const SceneRec& r = pa.recs[ONE ? 0u : find_rec<true>(pa.recs, pa.n_recs, q)];
     const uint32_t px = q - r.pix_off;
     const uint32_t WH = r.WH, S = r.spp_pass;
     const float4* Ls = L + r.path_off;
@@ -358,14 +364,16 @@ ShadeBufs shade_bufs(Workspace& ws, int cur) {
 
 Status launch_raygen(const Pass& p, Workspace& ws, hipStream_t s) {
     const uint32_t blocks = std::max<uint32_t>(1, (p.n_paths + kBlock - 1) / kBlock);
-    k_raygen<<<blocks, kBlock, 0, s>>>(p, ws.q_o[0].as<float4>(), ws.q_d[0].as<float4>(), ws.q_b[0].as<float4>(),
-                                       ws.L.as<float4>(), ws.counters.as<uint32_t>());
+    auto k = p.n_recs == 1 ? k_raygen<true> : k_raygen<false>;
+    k<<<blocks, kBlock, 0, s>>>(p, ws.q_o[0].as<float4>(), ws.q_d[0].as<float4>(), ws.q_b[0].as<float4>(),
+                                ws.L.as<float4>(), ws.counters.as<uint32_t>());
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }
 
 Status launch_shade(const Pass& p, Workspace& ws, int cur, uint32_t depth, int grid, hipStream_t s) {
-    k_shade<<<grid, kBlock, 0, s>>>(p, shade_bufs(ws, cur), depth);
+    auto k = p.n_recs == 1 ? k_shade<true> : k_shade<false>;
+    k<<<grid, kBlock, 0, s>>>(p, shade_bufs(ws, cur), depth);
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }
@@ -376,15 +384,17 @@ Status launch_tail(const Pass& p, Workspace& ws, int cur, uint32_t depth, int gr
     b.no = b.nd = b.nb = nullptr;
     const size_t need = (size_t)grid * kBlock * (kTravMaxStack - kTravLds) * 4;
     if (need > ws.spill.bytes) return Status{AURORA_ERR_RUNTIME, "tail grid exceeds the stack spill buffer"};
-    k_tail<<<grid, kBlock, 0, s>>>(p, b, depth, ws.spill.as<uint32_t>(), (uint32_t)(grid * kBlock));
+    auto k = p.n_recs == 1 ? k_tail<true> : k_tail<false>;
+    k<<<grid, kBlock, 0, s>>>(p, b, depth, ws.spill.as<uint32_t>(), (uint32_t)(grid * kBlock));
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }
 
 Status launch_accumulate(const Pass& p, Workspace& ws, hipStream_t s) {
     if (p.n_pixels == 0) return Status::ok();
-    k_accumulate<<<(p.n_pixels + kBlock - 1) / kBlock, kBlock, 0, s>>>(p, ws.L.as<float4>(), ws.aov_a.as<float4>(),
-                                                                       ws.aov_n.as<float4>());
+    auto k = p.n_recs == 1 ? k_accumulate<true> : k_accumulate<false>;
+    k<<<(p.n_pixels + kBlock - 1) / kBlock, kBlock, 0, s>>>(p, ws.L.as<float4>(), ws.aov_a.as<float4>(),
+                                                            ws.aov_n.as<float4>());
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }

@@ -29,7 +29,10 @@ constexpr int kParts = 8;
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kNoPrim = 0x7fffffffu;
 
-template <bool COUNT>
+// ONE: every ray belongs to the first scene record, whose fields are then
+// wave-uniform (scalar registers, loaded once); otherwise each claimed ray
+// loads its scene's record.
+template <bool COUNT, bool ONE>
 __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
     __shared__ uint32_t stk[kLds][kBlock];
     const uint32_t n = a.count ? *a.count : a.n;
@@ -40,12 +43,22 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
     uint32_t* spill = a.spill + (size_t)blockIdx.x * kBlock + tid;
     const size_t sstride = a.spill_stride;
     const float tmax = a.tmax;
-    float tmin = 0.0f;  // the scene's ray offset, loaded with the ray
-    // the scene of the lane's current ray (loaded when the ray is claimed)
+    float tmin = 0.0f;  // the scene's ray offset
+    // the scene of the lane's current ray
     const float4* N = nullptr;
     const float4* S = nullptr;
     const float4* Qd = nullptr;
     uint32_t n_spheres = 0, n_nodes = 0, n_prims = 0;
+    if constexpr (ONE) {
+        const SceneRec& rec = a.recs[0];
+        tmin = rec.tmin;
+        N = rec.sc.nodes;
+        S = rec.sc.spheres;
+        Qd = rec.sc.quads;
+        n_spheres = rec.sc.n_spheres;
+        n_nodes = rec.sc.n_nodes;
+        n_prims = rec.sc.n_spheres + rec.sc.n_quads;
+    }
 
     uint32_t ri = kNone;
     bool done = false;
@@ -126,18 +139,20 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
                     done = true;
                 } else if (rank < avail) {
                     ri = r_lo + b_next + rank;
+                    if (a.order) ri = a.order[ri];
                     const float4 ro = a.ray_o[ri], rd = a.ray_d[ri];
                     o = xyz(ro);
                     d = xyz(rd);
-                    const SceneRec& rec = a.recs[__float_as_uint(rd.w)];
-                    const DevScene& sc = rec.sc;
-                    tmin = rec.tmin;
-                    N = sc.nodes;
-                    S = sc.spheres;
-                    Qd = sc.quads;
-                    n_spheres = sc.n_spheres;
-                    n_nodes = sc.n_nodes;
-                    n_prims = sc.n_spheres + sc.n_quads;
+                    if constexpr (!ONE) {
+                        const SceneRec& rec = a.recs[__float_as_uint(rd.w)];
+                        tmin = rec.tmin;
+                        N = rec.sc.nodes;
+                        S = rec.sc.spheres;
+                        Qd = rec.sc.quads;
+                        n_spheres = rec.sc.n_spheres;
+                        n_nodes = rec.sc.n_nodes;
+                        n_prims = rec.sc.n_spheres + rec.sc.n_quads;
+                    }
                     inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
                     best_t = tmax;
                     best_id = kNoPrim;
@@ -250,15 +265,55 @@ __global__ __launch_bounds__(kBlock) void k_brute(DevScene sc, const float4* ro,
     hit[i] = make_float2(miss ? tmax : best_t, __int_as_float(miss ? AURORA_MISS : (int)best_id));
 }
 
+__device__ __forceinline__ uint32_t spread9(uint32_t v) {
+    // bits 0..8 of v to every third bit
+    v &= 0x1ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t cell9(float x, float lo, float hi) {
+    const float e = hi - lo;
+    const float u = e > 0.0f ? (x - lo) / e : 0.0f;
+    return (uint32_t)fminf(fmaxf(u * 512.0f, 0.0f), 511.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ray_keys(const float4* ro, const float4* rd, uint32_t n,
+                                                     const float4* nodes, uint32_t* keys, uint32_t* vals) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 lo = nodes[0], hi = nodes[1];
+    const float4 o = ro[i], d = rd[i];
+    const uint32_t m = (spread9(cell9(o.x, lo.x, hi.x)) << 2) | (spread9(cell9(o.y, lo.y, hi.y)) << 1) |
+                       spread9(cell9(o.z, lo.z, hi.z));
+    const uint32_t oct = (d.x < 0.0f ? 4u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 1u : 0u);
+    keys[i] = (m << 3) | oct;
+    vals[i] = i;
+}
+
 }  // namespace
+
+Status launch_ray_keys(const float4* ro, const float4* rd, uint32_t n, const float4* nodes, uint32_t* keys,
+                       uint32_t* vals, hipStream_t s) {
+    if (n == 0) return Status::ok();
+    k_ray_keys<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(ro, rd, n, nodes, keys, vals);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
 
 int extend_block() { return kBlock; }
 int extend_spill_levels() { return kMaxStack - kLds; }
 
 Status launch_extend(const ExtendArgs& a, int grid, hipStream_t s) {
     if ((size_t)grid * kBlock > a.spill_stride) return Status{AURORA_ERR_RUNTIME, "extend grid exceeds spill stride"};
-    if (a.counts || a.totals) k_extend<true><<<grid, kBlock, 0, s>>>(a);
-    else k_extend<false><<<grid, kBlock, 0, s>>>(a);
+    const bool count = a.counts || a.totals;
+    if (count && a.one_scene) k_extend<true, true><<<grid, kBlock, 0, s>>>(a);
+    else if (count) k_extend<true, false><<<grid, kBlock, 0, s>>>(a);
+    else if (a.one_scene) k_extend<false, true><<<grid, kBlock, 0, s>>>(a);
+    else k_extend<false, false><<<grid, kBlock, 0, s>>>(a);
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }

@@ -18,6 +18,12 @@ constexpr int kTravMaxStack = 64;  // Karras trees are shallower than this
 __device__ __forceinline__ void traverse_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
                                                  uint32_t* stk, int col_stride, uint32_t* spill, size_t sstride,
                                                  float& t_out, int& prim_out) {
+    // scene fields in registers: the spill stores below could otherwise
+    // force a reload of every field read through `sc` after them
+    const float4* const N = sc.nodes;
+    const float4* const S = sc.spheres;
+    const float4* const Qd = sc.quads;
+    const uint32_t n_spheres = sc.n_spheres, n_nodes = sc.n_nodes;
     const uint32_t n_prims = sc.n_spheres + sc.n_quads;
     const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float best_t = tmax;
@@ -26,10 +32,10 @@ __device__ __forceinline__ void traverse_closest(const DevScene& sc, v3 o, v3 d,
         if (id >= n_prims) return;
         float t;
         bool h;
-        if (id < sc.n_spheres) {
-            h = hit_sphere(sc.spheres[id], o, d, tmin, tmax, t);
+        if (id < n_spheres) {
+            h = hit_sphere(S[id], o, d, tmin, tmax, t);
         } else {
-            const float4* q = sc.quads + 3 * (size_t)(id - sc.n_spheres);
+            const float4* q = Qd + 3 * (size_t)(id - n_spheres);
             float al, be;
             h = hit_quad(q[0], q[1], q[2], o, d, tmin, tmax, t, al, be);
         }
@@ -38,8 +44,7 @@ __device__ __forceinline__ void traverse_closest(const DevScene& sc, v3 o, v3 d,
             best_id = id;
         }
     };
-    if (sc.n_nodes > 0) {
-        const float4* N = sc.nodes;
+    if (n_nodes > 0) {
         const float4 r0 = N[0], r1 = N[1];
         float tr;
         const uint32_t rc = __float_as_uint(r0.w);
@@ -50,7 +55,7 @@ __device__ __forceinline__ void traverse_closest(const DevScene& sc, v3 o, v3 d,
                 uint32_t cur = rc;
                 int sp = 0;
                 for (;;) {
-                    if ((cur & 1u) || cur + 1 >= sc.n_nodes) break;
+                    if ((cur & 1u) || cur + 1 >= n_nodes) break;
                     const float4 a0 = N[2 * (size_t)cur], a1 = N[2 * (size_t)cur + 1];
                     const float4 b0 = N[2 * (size_t)cur + 2], b1 = N[2 * (size_t)cur + 3];
                     float ta, tb;

@@ -158,8 +158,8 @@ Status Unet::backward(const float* target, float* loss) {
     AUR_TRY(dcat_.alloc(max_cat * 4));
     AUR_TRY(grads_.alloc(n_params_ * 4));
     const unsigned nb = (unsigned)((npix + 255) / 256);
-    AUR_TRY(loss_grad(act_[kOut].as<float>(), noisy_log_.as<float>(), target, npix, grad_[kOut].as<float>(),
-                      loss_part_.as<float>(), s_));
+    AUR_TRY(loss_grad(act_[kOut].as<float>(), noisy_log_.as<float>(), target, npix, loss_kind_,
+                      grad_[kOut].as<float>(), loss_part_.as<float>(), s_));
     for (int li = (int)layers_.size() - 1; li >= 0; --li) {
         const Layer& L = layers_[li];
         const int h = H >> L.level, w = W >> L.level;

@@ -26,8 +26,8 @@ Status split_concat_grad(const float* d, int N, int H, int W, int ca, int cb, fl
 Status prep_input(const float* rgb, const float* alb, const float* nrm, size_t npix, float inv_spp, uint16_t* x,
                   float* noisy_log, hipStream_t s);
 Status output_image(const float* out16, const float* noisy_log, size_t npix, float* img, hipStream_t s);
-Status loss_grad(const float* out16, const float* noisy_log, const float* target, size_t npix, float* g16f,
-                 float* loss_partial, hipStream_t s);
+Status loss_grad(const float* out16, const float* noisy_log, const float* target, size_t npix, int kind,
+                 float* g16f, float* loss_partial, hipStream_t s);
 Status adam_step(float* w, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2, float eps,
                  int t, hipStream_t s);
 Status pack_weights(const float* w, int cout, int cin, int kpad, int kpad_t, uint16_t* wk, uint16_t* wt,
@@ -59,6 +59,7 @@ class Unet {
     // loss against targets (mean radiance) and gradients into grads_
     Status backward(const float* target, float* loss);
     Status adam(float lr);
+    void set_loss(int kind) { loss_kind_ = kind; }  // 0 log-space L2, 1 relative L2
     Status grads_to_host(float* host);
     void release();
 
@@ -71,6 +72,7 @@ class Unet {
     DevBuf params_, grads_, m_, v_;
     int N_ = 0, H_ = 0, W_ = 0;
     int t_ = 0;
+    int loss_kind_ = 0;
     static constexpr int kSlots = 19;  // x, 14 hidden, 3 pools, out
     DevBuf act_[kSlots];               // BF16 activations (slot 18: FP32 out16)
     DevBuf grad_[kSlots];              // FP32 gradients of activations

@@ -308,10 +308,15 @@ __global__ void k_output(const float* __restrict__ out16, const float* __restric
     for (int c = 0; c < 3; ++c) img[3 * p + c] = fmaxf(expm1f(noisy_log[3 * p + c] + out16[16 * p + c]), 0.0f);
 }
 
-// loss = mean over pixels and channels of (pred - log(1 + target))^2 in
-// log space; g16 = dloss/dout (bf16), per-block partial loss
+// kind 0: loss = mean over pixels and channels of (z - log(1 + target))^2,
+//   z = log(1 + noisy) + out, the prediction in log space;
+// kind 1: loss = mean of (D - T)^2 / (D^2 + 0.01), D = expm1(z), with the
+//   denominator held constant in the gradient: the relative MSE of the
+//   error metric, unbiased under noisy targets (Lehtinen et al. 2018,
+//   Noise2Noise); dD/dz = D + 1.
+// g16 = dloss/dout (FP32, 16 channels), per-block partial loss
 __global__ void k_loss(const float* __restrict__ out16, const float* __restrict__ noisy_log,
-                       const float* __restrict__ target, size_t npix, float* __restrict__ g16f,
+                       const float* __restrict__ target, size_t npix, int kind, float* __restrict__ g16f,
                        float* __restrict__ loss_partial) {
     __shared__ float red[256];
     const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -322,9 +327,19 @@ __global__ void k_loss(const float* __restrict__ out16, const float* __restrict_
         for (int c = 0; c < 16; ++c) {
             float gv = 0.0f;
             if (c < 3) {
-                const float d = (noisy_log[3 * p + c] + out16[16 * p + c]) - log1pf(fmaxf(target[3 * p + c], 0.0f));
-                l += d * d;
-                gv = scale * d;
+                const float z = noisy_log[3 * p + c] + out16[16 * p + c];
+                const float t = fmaxf(target[3 * p + c], 0.0f);
+                if (kind == 0) {
+                    const float d = z - log1pf(t);
+                    l += d * d;
+                    gv = scale * d;
+                } else {
+                    const float D = expm1f(z);
+                    const float e = D - t;
+                    const float inv = 1.0f / (D * D + 0.01f);
+                    l += e * e * inv;
+                    gv = scale * e * inv * (D + 1.0f);
+                }
             }
             g16f[16 * p + c] = gv;
         }
@@ -486,9 +501,10 @@ Status output_image(const float* out16, const float* noisy_log, size_t npix, flo
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }
-Status loss_grad(const float* out16, const float* noisy_log, const float* target, size_t npix, float* g16f,
-                 float* loss_partial, hipStream_t s) {
-    k_loss<<<nblocks(npix), 256, 0, s>>>(out16, noisy_log, target, npix, g16f, loss_partial);
+Status loss_grad(const float* out16, const float* noisy_log, const float* target, size_t npix, int kind,
+                 float* g16f, float* loss_partial, hipStream_t s) {
+    if (kind != 0 && kind != 1) return Status{AURORA_ERR_INVALID, "unknown loss kind"};
+    k_loss<<<nblocks(npix), 256, 0, s>>>(out16, noisy_log, target, npix, kind, g16f, loss_partial);
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }

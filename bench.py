@@ -61,17 +61,26 @@ def stress_section(ctx, n_rays=1 << 22, render_spp=1):
     o, d = scenes.random_rays_torch(n_rays, [-0.05] * 3, [1.05] * 3, seed=0)
     prim = torch.empty(n_rays, dtype=torch.int32, device="cuda")
     t = torch.empty(n_rays, dtype=torch.float32, device="cuda")
-    scene.trace_device_timed(o, d, prim, t, tmin=1e-4)  # warm-up
-    runs = [scene.trace_device_timed(o, d, prim, t, tmin=1e-4) for _ in range(3)]
-    ms = sum(r["extend_ms"] for r in runs) / len(runs)
     cnt = scene.trace_device_timed(o, d, prim, t, tmin=1e-4, count_fetches=True)
     alg = 32 * cnt["nodes_fetched"] + 16 * cnt["spheres_fetched"] + 48 * cnt["quads_fetched"]
-    trav = {"rays": n_rays, "extend_ms": ms, "rays_per_s": n_rays / (ms * 1e-3),
-            "nodes_per_ray": cnt["nodes_fetched"] / n_rays,
-            "prims_per_ray": (cnt["spheres_fetched"] + cnt["quads_fetched"]) / n_rays,
-            "alg_bytes": alg, "achieved_GBs": alg / (ms * 1e-3) / 1e9,
-            "frac_of_hbm_peak": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "hit_fraction": float((prim >= 0).float().mean().item())}
+
+    def timed(sort_mode):
+        # extend_ms brackets the ray sort (when on) and the traversal
+        ctx.set_ray_sort(sort_mode)
+        scene.trace_device_timed(o, d, prim, t, tmin=1e-4)  # warm-up
+        runs = [scene.trace_device_timed(o, d, prim, t, tmin=1e-4) for _ in range(3)]
+        ctx.set_ray_sort(1)
+        ms = sum(r["extend_ms"] for r in runs) / len(runs)
+        note(f"stress traversal (ray sort {sort_mode}): {ms:.1f} ms for {n_rays} rays")
+        return {"rays": n_rays, "ray_sort": sort_mode, "extend_ms": ms, "rays_per_s": n_rays / (ms * 1e-3),
+                "nodes_per_ray": cnt["nodes_fetched"] / n_rays,
+                "prims_per_ray": (cnt["spheres_fetched"] + cnt["quads_fetched"]) / n_rays,
+                "alg_bytes": alg, "achieved_GBs": alg / (ms * 1e-3) / 1e9,
+                "frac_of_hbm_peak": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "hit_fraction": float((prim >= 0).float().mean().item())}
+
+    trav = timed(1)
+    trav_unsorted = timed(0)
     cam = scenes.stress_camera(1920, 1080)
     rgb = torch.zeros((1080, 1920, 3), dtype=torch.float32, device="cuda")
     rs = RenderSettings(spp=render_spp, max_depth=16, seed=0, background=(0.7, 0.8, 1.0), tmin=1e-4)
@@ -88,7 +97,8 @@ def stress_section(ctx, n_rays=1 << 22, render_spp=1):
               "extend_ms": st["extend_ms"], "extend_achieved_GBs": ralg / (st["extend_ms"] * 1e-3) / 1e9}
     scene.close()
     return {"workload": "stress_2M_spheres_2M_triangles", "bvh_nodes": bvh["n_nodes"], "bvh_depth": bvh["depth"],
-            "bvh_build_ms": bvh["build_ms"], "traversal": trav, "render_1920x1080_depth16": render,
+            "bvh_build_ms": bvh["build_ms"], "traversal": trav, "traversal_unsorted": trav_unsorted,
+            "render_1920x1080_depth16": render,
             "setup_s": time.perf_counter() - t0}
 
 

@@ -202,6 +202,14 @@ void* aurora_stream(aurora_ctx* ctx);
 int aurora_set_launch(aurora_ctx* ctx, int extend_blocks_per_cu,
                       int shade_blocks_per_cu, uint32_t tail_paths);
 
+/* Order of the rays of aurora_trace_closest*: 0 = as given, 1 = sorted by
+ * origin (Morton code in the root box) and direction octant when the query
+ * has at least 2^20 rays and the tree at least 2^21 nodes (default), 2 =
+ * always sorted.  The hit of every ray is the same in every mode; sorting
+ * makes the rays traced together on one XCD share the nodes its L2 caches.
+ * No reference counterpart. */
+int aurora_set_ray_sort(aurora_ctx* ctx, int mode);
+
 /* Validate and upload a flattened scene (replaces are::ObjectSet,
  * include/object/object_set.h:10-12, with its Triangle, Material and Texture
  * pointers, include/object/triangle.h:38-39).  Rejects degenerate quads the
@@ -324,9 +332,17 @@ int aurora_denoise(aurora_denoiser* dn, const float* d_rgb_sum,
                    const float* d_albedo_sum, const float* d_normal_sum,
                    uint32_t spp, uint32_t width, uint32_t height,
                    float* d_out);
-/* Loss (mean squared error in log(1 + radiance) against d_target, linear
- * RGB means, batch x H x W x 3) and its gradient; host_grads (optional)
- * receives the flat FP32 gradient. */
+/* Training loss against d_target (linear RGB means, batch x H x W x 3).
+ * AURORA_LOSS_LOG_L2 (default): mean squared error in log(1 + radiance).
+ * AURORA_LOSS_REL_L2: mean of (D - T)^2 / (D^2 + 0.01) over pixels and
+ * channels, D the denoised radiance, with the denominator held constant in
+ * the gradient (the relative MSE of the error metric; unbiased with noisy
+ * targets). */
+#define AURORA_LOSS_LOG_L2 0
+#define AURORA_LOSS_REL_L2 1
+int aurora_denoiser_set_loss(aurora_denoiser* dn, int kind);
+/* Loss (as set by aurora_denoiser_set_loss) and its gradient; host_grads
+ * (optional) receives the flat FP32 gradient. */
 int aurora_denoiser_loss_grad(aurora_denoiser* dn, const float* d_rgb_sum,
                               const float* d_albedo_sum,
                               const float* d_normal_sum, const float* d_target,

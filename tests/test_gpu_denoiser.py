@@ -54,9 +54,11 @@ def test_forward_matches_torch(ctx):
     assert err.mean() < 2e-3
 
 
-def test_gradients_match_torch_autograd(ctx):
-    print("denoiser gradients", flush=True)
+@pytest.mark.parametrize("kind", ["log", "rel"])
+def test_gradients_match_torch_autograd(ctx, kind):
+    print("denoiser gradients", kind, flush=True)
     net = dn.Denoiser(ctx, seed=2)
+    net.set_loss(kind)
     p = random_params(net.n, 2)
     net.set_params(p)
     B, H, W, spp = 2, 16, 32, 4
@@ -65,7 +67,13 @@ def test_gradients_match_torch_autograd(ctx):
     pt = torch.from_numpy(p).to(torch.float64).requires_grad_(True)
     out16, noisy_log = dn.torch_reference(pt, rgb, alb, nrm, spp, emulate_bf16=False)
     pred = noisy_log + out16[:, :3]
-    ref_loss = ((pred - torch.log1p(tgt.to(torch.float64)).permute(0, 3, 1, 2)) ** 2).mean()
+    t64 = tgt.to(torch.float64).permute(0, 3, 1, 2)
+    if kind == "log":
+        ref_loss = ((pred - torch.log1p(t64)) ** 2).mean()
+    else:
+        # relative L2 with the denominator held constant (detached)
+        D = torch.expm1(pred)
+        ref_loss = ((D - t64) ** 2 / (D.detach() ** 2 + 0.01)).mean()
     ref_loss.backward()
     gr = pt.grad.numpy()
     cos = float(g @ gr / (np.linalg.norm(g) * np.linalg.norm(gr)))

@@ -91,6 +91,18 @@ def test_closest_hit_bit_exact(ctx, name, nrays):
     assert np.array_equal(gp, op), f"{(gp != op).sum()} prim mismatches"
     assert np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
     assert np.array_equal(gc, oc)
+    # traversal in sorted ray order (origin Morton code, octant) and in the
+    # given order give the same hits and fetch counts
+    ctx.set_ray_sort(2)
+    try:
+        dp.zero_(); dt.zero_(); dc.zero_()
+        g.trace_device(dev(org), dev(dirs), dp, dt, tmin=1e-4, counts=dc)
+        ctx.synchronize()
+    finally:
+        ctx.set_ray_sort(1)
+    assert np.array_equal(dp.cpu().numpy(), op)
+    assert np.array_equal(dt.cpu().numpy().view(np.uint32), ot.view(np.uint32))
+    assert np.array_equal(dc.cpu().numpy().astype(np.uint32), oc)
     # host-buffer entry point agrees with the device one
     hp, ht = g.trace(org[:4096], dirs[:4096], tmin=1e-4)
     assert np.array_equal(hp, op[:4096]) and np.array_equal(ht, ot[:4096])
