@@ -3,7 +3,7 @@
 // Pipeline (one stream, no host round trips):
 //   k_bounds_init -> k_prim_bounds (padded primitive boxes + wave-reduced
 //   scene/centroid bounds, ordered-u32 atomics) -> k_morton (30-bit codes)
-//   -> 4 x {k_hist, k_scan, k_scatter} (stable LSD radix sort of (code, id)
+//   -> 4 x {k_hist, scan, k_scatter} (stable LSD radix sort of (code, id)
 //   with 8-bit digits; the scatter ranks keys with wave64 ballots and LDS
 //   per-wave digit counts) -> k_karras (Karras 2012 radix tree) -> k_refit
 //   (bottom-up box union, one acquire/release arrival counter per node).
@@ -168,27 +168,104 @@ __global__ __launch_bounds__(kBlock) void k_hist(const uint32_t* keys, uint32_t 
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// exclusive scan of `total` counts by one block of 1024 threads
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* h, uint32_t total) {
-    __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t chunk = (total + 1023u) / 1024u;
-    const uint32_t b = t * chunk, e = min(total, b + chunk);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += h[i];
-    part[t] = s;
+// Exclusive scan of `total` counts in three coalesced kernels: per-tile
+// sums (kScanTile entries per block), a scan of the tile sums by one block,
+// then each tile rescanned through LDS with its offset added.
+constexpr int kScanItems = 16;
+constexpr uint32_t kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* red) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = t >= off ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    const uint32_t t = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return t;
+}
+
+// exclusive scan across the 256 threads of a block; returns the block total
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t& excl, uint32_t* red) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(inc, o);
+        if (lane >= o) inc += u;
     }
-    uint32_t run = part[t] - s;  // exclusive prefix of this chunk
-    for (uint32_t i = b; i < e; ++i) {
-        uint32_t c = h[i];
-        h[i] = run;
+    if (lane == 63) red[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        before += w < wave ? red[w] : 0u;
+        tot += red[w];
+    }
+    __syncthreads();
+    excl = before + inc - v;
+    return tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* h, uint32_t total, uint32_t* part) {
+    __shared__ uint32_t red[4];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    uint32_t v = 0;
+#pragma unroll
+    for (int it = 0; it < kScanItems; ++it) {
+        const size_t i = base + (size_t)it * kBlock + threadIdx.x;
+        if (i < total) v += h[i];
+    }
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// one block: exclusive scan of the np tile sums, 256 at a time with a carry
+__global__ __launch_bounds__(kBlock) void k_scan_parts(uint32_t* part, uint32_t np) {
+    __shared__ uint32_t red[4];
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < np; b += kBlock) {
+        const uint32_t i = b + threadIdx.x;
+        const uint32_t v = i < np ? part[i] : 0u;
+        uint32_t ex;
+        const uint32_t tot = block_exclusive(v, ex, red);
+        if (i < np) part[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* h, uint32_t total, const uint32_t* part) {
+    // row r of the tile at tile[r * 17]: the pad spreads a thread's 16
+    // consecutive entries over distinct banks
+    __shared__ uint32_t tile[kBlock * (kScanItems + 1)];
+    __shared__ uint32_t red[4];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    const int tid = threadIdx.x;
+    auto at = [](uint32_t k) { return (k / kScanItems) * (kScanItems + 1) + k % kScanItems; };
+#pragma unroll
+    for (int it = 0; it < kScanItems; ++it) {
+        const uint32_t k = it * kBlock + tid;
+        const size_t i = base + k;
+        tile[at(k)] = i < total ? h[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) sum += tile[tid * (kScanItems + 1) + j];
+    uint32_t run;
+    block_exclusive(sum, run, red);
+    run += part[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint32_t c = tile[tid * (kScanItems + 1) + j];
+        tile[tid * (kScanItems + 1) + j] = run;
         run += c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kScanItems; ++it) {
+        const uint32_t k = it * kBlock + tid;
+        const size_t i = base + k;
+        if (i < total) h[i] = tile[at(k)];
     }
 }
 
@@ -340,9 +417,23 @@ __global__ __launch_bounds__(kBlock) void k_depth(uint32_t n, const uint32_t* le
 
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
+// histogram entries of a sort of n keys, and the tile sums of its scan after them
+inline uint32_t hist_entries(uint32_t n) { return 256u * blocks_for(n, kTile); }
+
+void scan_exclusive(uint32_t* h, uint32_t total, hipStream_t st) {
+    uint32_t* part = h + total;
+    const uint32_t np = blocks_for(total, kScanTile);
+    k_scan_reduce<<<np, kBlock, 0, st>>>(h, total, part);
+    k_scan_parts<<<1, kBlock, 0, st>>>(part, np);
+    k_scan_apply<<<np, kBlock, 0, st>>>(h, total, part);
+}
+
 }  // namespace
 
-size_t radix_hist_bytes(uint32_t n) { return (size_t)256 * blocks_for(n, kTile) * 4; }
+size_t radix_hist_bytes(uint32_t n) {
+    const uint32_t e = hist_entries(n);
+    return ((size_t)e + blocks_for(e, kScanTile)) * 4;
+}
 
 Status radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, uint32_t n,
                         int passes, hipStream_t st) {
@@ -353,7 +444,7 @@ Status radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, 
     for (int pass = 0; pass < passes; ++pass) {
         const int a = pass & 1, b = a ^ 1;
         k_hist<<<nb, kBlock, 0, st>>>(ks[a], n, 8 * pass, hist, nb);
-        k_scan<<<1, 1024, 0, st>>>(hist, 256u * nb);
+        scan_exclusive(hist, 256u * nb, st);
         k_scatter<<<nb, kBlock, 0, st>>>(ks[a], vs[a], ks[b], vs[b], n, 8 * pass, hist, nb);
     }
     AUR_HIP(hipGetLastError());
@@ -379,7 +470,7 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& mort
         AUR_TRY(s.keys[k].alloc((size_t)n * 4));
         AUR_TRY(s.vals[k].alloc((size_t)n * 4));
     }
-    AUR_TRY(s.hist.alloc((size_t)256 * nb * 4));
+    AUR_TRY(s.hist.alloc(radix_hist_bytes(n)));
     const size_t ni = n > 1 ? n - 1 : 1;
     AUR_TRY(s.gamma.alloc(ni * 4));
     AUR_TRY(s.flags.alloc(ni * 4));
@@ -403,7 +494,7 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& mort
     for (int pass = 0; pass < 4; ++pass) {
         const int a = pass & 1, b = a ^ 1;
         k_hist<<<nb, kBlock, 0, st>>>(s.keys[a].as<uint32_t>(), n, 8 * pass, s.hist.as<uint32_t>(), nb);
-        k_scan<<<1, 1024, 0, st>>>(s.hist.as<uint32_t>(), 256u * nb);
+        scan_exclusive(s.hist.as<uint32_t>(), 256u * nb, st);
         k_scatter<<<nb, kBlock, 0, st>>>(s.keys[a].as<uint32_t>(), s.vals[a].as<uint32_t>(), s.keys[b].as<uint32_t>(),
                                          s.vals[b].as<uint32_t>(), n, 8 * pass, s.hist.as<uint32_t>(), nb);
     }

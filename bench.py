@@ -143,7 +143,8 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=4096):
     acc = buf()
     done, t_plain, curve = 0, 0.0, []
     err = float("inf")
-    for cp in [1024 * k for k in (1, 2, 4, 8, 12, 16, 20, 24, 28, 32, 40, 48)]:
+    # coarse checkpoints, then every 1024 spp where plain crosses 1e-3
+    for cp in [1024 * k for k in (1, 2, 4, 8, 12, 16)] + [1024 * k for k in range(17, 65)]:
         t_plain += render_into(acc, cp - done, done)
         done = cp
         err = unbiased_relmse(acc / done, A, B)
@@ -158,7 +159,7 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=4096):
         dn.load(WEIGHTS)
         dcurve = []
         best = None
-        for spp in (64, 128, 256, 512, 1024, 2048, 4096):
+        for spp in (32, 64, 96, 128, 160, 192, 224, 256, 384, 512, 1024, 2048, 4096):
             rgb, alb, nrm = buf(), buf(), buf()
             tr = render_into(rgb, spp, 0, alb, nrm)
             t0 = time.perf_counter()
