@@ -134,10 +134,11 @@ class RenderStats(C.Structure):
         ("passes", C.c_uint32),
         ("max_live", C.c_uint32),
         ("extend_launches", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("tail_launches", C.c_uint32),
         ("nodes_fetched", C.c_uint64),
         ("spheres_fetched", C.c_uint64),
         ("quads_fetched", C.c_uint64),
+        ("tail_ms", C.c_double),
     ]
 
 

@@ -18,7 +18,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "aurora_amd", "a
 CANDIDATES = {
     "extend_blocks_per_cu": [4, 8, 16],
     "shade_blocks_per_cu": [8],
-    "tail_paths": [1 << 18, 1 << 20, 1 << 22],
+    "tail_paths": [1 << 20, (1 << 32) - 1],
     "graph": [False, True],
 }
 

@@ -17,6 +17,11 @@ import numpy as np
 from . import abi
 
 
+# fused-tail switch above any queue length: every path of a pass runs in the
+# persistent tail kernel (the default; 0 = wavefront kernels only)
+TAIL_ALWAYS = (1 << 32) - 1
+
+
 @dataclass
 class SceneArrays:
     spheres: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.float32))
@@ -172,7 +177,7 @@ class Context:
         except Exception:
             pass
 
-    def set_launch(self, extend_blocks_per_cu=8, shade_blocks_per_cu=8, tail_paths=1 << 20):
+    def set_launch(self, extend_blocks_per_cu=8, shade_blocks_per_cu=8, tail_paths=TAIL_ALWAYS):
         self.check(self.lib.aurora_set_launch(self.h, extend_blocks_per_cu, shade_blocks_per_cu, tail_paths))
 
     def set_ray_sort(self, mode=1):

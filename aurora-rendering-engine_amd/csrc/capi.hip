@@ -23,6 +23,9 @@ using namespace aur;
 // automatic ray sorting of ray queries: at least this many rays on a tree of
 // at least this many nodes (32 bytes each: 64 MiB, beyond what the L2s hold)
 constexpr uint32_t kSortMinRays = 1u << 20;
+// resident 256-thread blocks of the fused tail kernel per CU (it is built
+// for five waves per SIMD)
+constexpr int kTailBlocksPerCu = 5;
 constexpr uint32_t kSortMinNodes = 1u << 21;
 
 struct aurora_ctx {
@@ -32,7 +35,7 @@ struct aurora_ctx {
     int cus = 0;
     int ext_grid = 0;    // resident blocks of the extend kernel
     int shade_grid = 0;
-    uint32_t tail_paths = 1u << 20;  // fused-tail switch: live paths at or below
+    uint32_t tail_paths = 0xffffffffu;  // fused-tail switch: live paths at or below (default: always)
     uint32_t graph_tail_depth = 8;   // captured passes switch to the fused tail here
     int ray_sort = 1;                // ray queries: 0 never sort, 1 large scenes, 2 always
     Workspace ws;                    // path queues shared by the context's renders
@@ -427,8 +430,9 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         return Status::ok();
     }
     const int ext_grid = c->ext_grid;
-    const int tail_grid = (int)std::min<uint64_t>(4ull * ext_grid, std::max<uint64_t>(1, (std::min<uint64_t>(c->tail_paths, cap) + 255) / 256));
-    const int graph_tail_grid = 4 * ext_grid;
+    // the fused tail is persistent: its grid is the resident capacity
+    const int tail_grid = c->cus * kTailBlocksPerCu;
+    const int graph_tail_grid = tail_grid;
     AUR_TRY(ensure_workspace(c, (uint32_t)cap, std::max(ext_grid, graph ? graph_tail_grid : tail_grid)));
     Workspace& w = c->ws;
     AUR_TRY(w.recs.alloc(jobs.size() * sizeof(SceneRec)));
@@ -461,8 +465,10 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         AUR_TRY(ev(t_end));
         AUR_HIP(hipEventRecord(t_begin, c->stream));
     }
-    std::vector<hipEvent_t> ext_ev;
+    std::vector<hipEvent_t> ext_ev, tail_ev;
     std::vector<hipEvent_t> cnt_ev(max_depth);
+    // counting runs trace every segment in k_extend, whose counters they read
+    const uint32_t tail_paths = count_fetches ? 0u : c->tail_paths;
     for (auto& e2 : cnt_ev) AUR_TRY(ev(e2));
     uint64_t total_segments = 0;
 
@@ -562,11 +568,13 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
             }
             AUR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             Status cs = launch_raygen(pa, w, c->stream);
-            const uint32_t dt = std::min(max_depth, c->graph_tail_depth);
+            // captured passes run a fixed number of wavefront bounces, then
+            // the fused tail; none when the whole pass fits the tail switch
+            const uint32_t dt = pa.n_paths <= tail_paths ? 0u : std::min(max_depth, c->graph_tail_depth);
             for (uint32_t depth = 0; depth < dt && !cs.bad(); ++depth) {
                 const int cur = depth & 1;
                 ExtendArgs a = extend_args(cur);
-                a.spill_stride = (uint32_t)(graph_tail_grid * extend_block());
+                a.spill_stride = (uint32_t)(std::max(ext_grid, graph_tail_grid) * extend_block());
                 cs = launch_extend(a, ext_grid, c->stream);
                 if (!cs.bad()) cs = launch_shade(pa, w, cur, depth, c->shade_grid, c->stream);
             }
@@ -595,9 +603,20 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
             }
             // one item per thread while the queue is shorter than the chip
             const uint64_t blocks = std::max<uint64_t>(1, ((uint64_t)ub + 255) / 256);
-            if (ub <= c->tail_paths) {
+            if (ub <= tail_paths) {
                 const int tg = (int)std::min<uint64_t>((uint64_t)tail_grid, blocks);
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (timed) {
+                    AUR_TRY(ev(e0));
+                    AUR_TRY(ev(e1));
+                    AUR_HIP(hipEventRecord(e0, c->stream));
+                }
                 AUR_TRY(launch_tail(pa, w, cur, depth, tg, c->stream));
+                if (timed) {
+                    AUR_HIP(hipEventRecord(e1, c->stream));
+                    tail_ev.push_back(e0);
+                    tail_ev.push_back(e1);
+                }
                 break;
             }
             const int eg = (int)std::min<uint64_t>((uint64_t)ext_grid, blocks);
@@ -641,6 +660,14 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         }
         st.extend_ms = ext;
         st.extend_launches = (uint32_t)(ext_ev.size() / 2);
+        double tail = 0.0;
+        for (size_t k = 0; k + 1 < tail_ev.size(); k += 2) {
+            float m2 = 0.0f;
+            AUR_HIP(hipEventElapsedTime(&m2, tail_ev[k], tail_ev[k + 1]));
+            tail += m2;
+        }
+        st.tail_ms = tail;
+        st.tail_launches = (uint32_t)(tail_ev.size() / 2);
     }
     if (count_fetches) {
         unsigned long long tot[3];

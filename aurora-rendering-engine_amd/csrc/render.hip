@@ -268,39 +268,93 @@ __global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t
     if (lane == 0 && segs) atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kTailSegs), segs);
 }
 
-// Fused tail: once the live queue is short, each thread carries one path
-// from segment `depth` to termination (traverse -> shade -> ...), so the
-// last bounces cost one launch instead of two per bounce.  Same arithmetic
-// as the extend/shade pair, so results are unchanged.
+// Fused tail: once the live queue is short, each lane carries one path to
+// termination (traverse -> shade -> ...), so the last bounces cost one
+// launch instead of two per bounce.  The grid is persistent: a lane whose
+// path ends takes the next one from the queue (paths claimed per wave in
+// batches from one of the per-XCD pools, dealt out by ballot rank), so a
+// wave is not held by its longest path.  Same arithmetic per path as the
+// extend/shade pair, so results are unchanged.
+constexpr int kTailRefill = 4;
+constexpr uint32_t kTailBatch = 64;
+
+// five waves per SIMD: the register budget this leaves (84) holds the kernel
+// without spills; the resident grid (five blocks per CU) matches it
 template <bool ONE>
-__global__ __launch_bounds__(kBlock) void k_tail(Pass pa, ShadeBufs b, uint32_t depth0, uint32_t* spill,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_tail(
+    Pass pa, ShadeBufs b, uint32_t depth0, uint32_t* spill,
                                                  uint32_t spill_stride) {
     __shared__ uint32_t stk[kTravLds * kBlock];
     const uint32_t n = b.counters[b.cur];
     uint32_t* my_spill = spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t parts = min((uint32_t)kPools, gridDim.x);
+    const uint32_t part = blockIdx.x % parts;
+    const uint32_t r_lo = (uint32_t)(((uint64_t)n * part) / parts);
+    const uint32_t r_n = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - r_lo;
+    uint32_t* pool = b.counters + kPoolBase + kPoolStride * part;
+    uint32_t b_next = 0, b_end = 0;  // wave-uniform, offsets within the range
+    bool has = false, done = false;
+    uint32_t path = 0, sid = 0, depth = 0;
+    v3 org = V(0, 0, 0), dir = V(0, 0, 0), beta = V(0, 0, 0);
     unsigned long long segs = 0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
-        const uint32_t path = __float_as_uint(O.w);
-        const SceneRec& r = pa.recs[ONE ? 0u : __float_as_uint(D.w)];
-        v3 org = xyz(O), dir = xyz(D), beta = xyz(B);
-        for (uint32_t depth = depth0; depth < r.max_depth; ++depth) {
-            ++segs;
-            float t;
-            int prim;
-            traverse_closest(r.sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill, spill_stride, t,
-                             prim);
-            const ShadeOut o = shade_core(r, b, org, dir, beta, t, prim, path, depth);
-            if (!o.alive) break;
+    for (;;) {
+        const bool idle = !has && !done;
+        const uint64_t im = __ballot(idle);
+        const uint64_t am = __ballot(has);
+        if (im != 0 && (am == 0 || __popcll(im) >= kTailRefill)) {
+            if (b_next >= b_end) {
+                uint32_t base = r_n;
+                if (lane == 0) {
+                    if (__hip_atomic_load(pool, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < r_n)
+                        base = atomicAdd(pool, kTailBatch);
+                }
+                base = __shfl(base, 0);
+                b_next = base;
+                b_end = base < r_n ? min(r_n, base + kTailBatch) : base;
+            }
+            const uint32_t avail = b_end - b_next;
+            const uint32_t rank = (uint32_t)__popcll(im & lt);
+            if (idle) {
+                if (avail == 0) {
+                    done = true;
+                } else if (rank < avail) {
+                    const uint32_t i = r_lo + b_next + rank;
+                    const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
+                    path = __float_as_uint(O.w);
+                    sid = ONE ? 0u : __float_as_uint(D.w);
+                    org = xyz(O);
+                    dir = xyz(D);
+                    beta = xyz(B);
+                    depth = depth0;
+                    has = true;
+                }
+            }
+            b_next += min(avail, (uint32_t)__popcll(im));
+        }
+        if (__ballot(has || !done) == 0) break;
+        if (!has) continue;
+        // one segment of this lane's path
+        const SceneRec& r = pa.recs[sid];
+        ++segs;
+        float t;
+        int prim;
+        traverse_closest(r.sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill, spill_stride, t,
+                         prim);
+        const ShadeOut o = shade_core(r, b, org, dir, beta, t, prim, path, depth);
+        if (!o.alive) {
+            has = false;
+        } else {
             org = o.p;
             dir = o.nd;
             beta = o.beta;
+            ++depth;
         }
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) segs += __shfl_xor(segs, off);
-    if ((threadIdx.x & 63) == 0 && segs)
-        atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kTailSegs), segs);
+    if (lane == 0 && segs) atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kTailSegs), segs);
 }
 
 // per pixel: add the pass's samples in sample order to the scene's sums

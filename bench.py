@@ -84,17 +84,24 @@ def stress_section(ctx, n_rays=1 << 22, render_spp=1):
     cam = scenes.stress_camera(1920, 1080)
     rgb = torch.zeros((1080, 1920, 3), dtype=torch.float32, device="cuda")
     rs = RenderSettings(spp=render_spp, max_depth=16, seed=0, background=(0.7, 0.8, 1.0), tmin=1e-4)
-    scene.render_device(cam, rs, rgb)
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    st = scene.render_device(cam, rs, rgb)
-    ctx.synchronize()
-    wall = time.perf_counter() - t1
     rc = scene.render_device(cam, RenderSettings(spp=render_spp, max_depth=16, seed=0, background=(0.7, 0.8, 1.0),
                                                  tmin=1e-4, count_fetches=True), rgb)
     ralg = 32 * rc["nodes_fetched"] + 16 * rc["spheres_fetched"] + 48 * rc["quads_fetched"]
-    render = {"paths": st["paths"], "paths_per_s": st["paths"] / wall, "rays_per_s": st["segments"] / wall,
-              "extend_ms": st["extend_ms"], "extend_achieved_GBs": ralg / (st["extend_ms"] * 1e-3) / 1e9}
+    render = {"alg_traversal_bytes": ralg}
+    for mode, tp in (("wavefront_tail_1M", 1 << 20), ("persistent_tail", TAIL_ALWAYS)):
+        ctx.set_launch(tail_paths=tp)
+        scene.render_device(cam, rs, rgb)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        st = scene.render_device(cam, rs, rgb)
+        ctx.synchronize()
+        wall = time.perf_counter() - t1
+        k_ms = st["extend_ms"] + st["tail_ms"]
+        render[mode] = {"paths": st["paths"], "paths_per_s": st["paths"] / wall, "rays_per_s": st["segments"] / wall,
+                        "extend_ms": st["extend_ms"], "tail_ms": st["tail_ms"],
+                        "traversal_kernels_achieved_GBs": ralg / (k_ms * 1e-3) / 1e9}
+        note(f"stress render {mode}: {st['paths'] / wall / 1e6:.1f} Mpaths/s")
+    ctx.set_launch()
     scene.close()
     return {"workload": "stress_2M_spheres_2M_triangles", "bvh_nodes": bvh["n_nodes"], "bvh_depth": bvh["depth"],
             "bvh_build_ms": bvh["build_ms"], "traversal": trav, "traversal_unsorted": trav_unsorted,
@@ -103,6 +110,13 @@ def stress_section(ctx, n_rays=1 << 22, render_spp=1):
 
 
 WEIGHTS = os.path.join(ROOT, "aurora-rendering-engine_amd", "weights", "unet_cornell.f32")
+TAIL_ALWAYS = (1 << 32) - 1  # fused-tail switch above any queue: one persistent kernel per pass
+
+# HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain 1 GiB copy
+# gives), keyed (kernel, width, tail switch); the summaries and commands
+# are in the profiles/ directory.
+PMC_TRAFFIC = {("k_tail", 1024, TAIL_ALWAYS): 667.9e6}  # profiles/tail_cornell_1024x1024.md
 
 
 def note(msg):
@@ -211,8 +225,8 @@ def serving_section(ctx, n_scenes=64, seed0=0):
 
     note(f"serving: {n_scenes} scenes uploaded ({setup:.1f} s); autotuning")
     best, table = autotune.tune(ctx, run, autotune.signature(items), cache_path=None, log=note,
-                                candidates={"extend_blocks_per_cu": [4, 8, 16], "shade_blocks_per_cu": [8],
-                                            "tail_paths": [1 << 18, 1 << 20], "graph": [False, True]})
+                                candidates={"extend_blocks_per_cu": [8, 16], "shade_blocks_per_cu": [8],
+                                            "tail_paths": [1 << 20, TAIL_ALWAYS], "graph": [False, True]})
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = run(best["graph"])
@@ -294,13 +308,36 @@ def main():
     S = args.spp_per_step
     rgb = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
 
+    graph = False
+
     def step(i, count=False):
         rs = RenderSettings(spp=S, sample_offset=i * S, max_depth=50, seed=0, spp_per_pass=S,
-                            count_fetches=count)
+                            count_fetches=count, graph=graph and not count)
         return scene.render_device(cam, rs, rgb)
 
+    # launch shape for this workload from the autotuner (untimed; the shape
+    # changes speed only, never the image)
+    tune_table = []
+    if args.tail_paths is None and not args.sweep_tail:
+        from aurora_amd import autotune
+
+        def tune_run(g):
+            nonlocal graph
+            graph = g
+            step(0)
+            ctx.synchronize()
+
+        best, tune_table = autotune.tune(ctx, tune_run, f"cornell_{W}x{H}_spp{S}", cache_path=None, log=note,
+                                         candidates={"extend_blocks_per_cu": [8], "shade_blocks_per_cu": [8],
+                                                     "tail_paths": [1 << 20, TAIL_ALWAYS],
+                                                     "graph": [False, True]})
+        graph = best["graph"]
+        shape = {k: best[k] for k in best if k != "seconds"}
+    else:
+        shape = {"tail_paths": args.tail_paths}
+
     if args.sweep_tail:
-        for tp in (0, 65536, 262144, 1 << 20, 4 << 20, 1 << 30):
+        for tp in (0, 262144, 1 << 20, 4 << 20, TAIL_ALWAYS):
             ctx.set_launch(tail_paths=tp)
             step(0)
             ctx.synchronize()
@@ -311,7 +348,7 @@ def main():
             dt = (time.perf_counter() - t0) / 5
             print(f"sweep tail_paths={tp}: {dt * 1e3:.2f} ms/step, {W * H * S / dt / 1e6:.1f} Mpaths/s",
                   file=sys.stderr, flush=True)
-        ctx.set_launch(tail_paths=args.tail_paths if args.tail_paths is not None else 1 << 20)
+        ctx.set_launch(tail_paths=args.tail_paths if args.tail_paths is not None else TAIL_ALWAYS)
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()
@@ -327,14 +364,30 @@ def main():
     paths = sum(s["paths"] for s in steps)
     segments = sum(s["segments"] for s in steps)
     ext_ms = sum(s["extend_ms"] for s in steps)
-    launches = sum(s["extend_launches"] for s in steps)
+    tail_ms = sum(s["tail_ms"] for s in steps)
     # algorithmic traversal bytes of the first timed step: 32 B per node,
-    # 16 B per sphere, 48 B per quad record fetched (counted by the same
-    # traversal, rerun untimed with counting on)
+    # 16 B per sphere, 48 B per quad record fetched, counted by the same
+    # traversal rerun untimed with counting on (every segment in k_extend)
     cnt = step(args.warmup, count=True)
-    alg_bytes = 32 * cnt["nodes_fetched"] + 16 * cnt["spheres_fetched"] + 48 * cnt["quads_fetched"]
+    alg_trav = 32 * cnt["nodes_fetched"] + 16 * cnt["spheres_fetched"] + 48 * cnt["quads_fetched"]
+    # kernel durations from HIP events on the library stream: captured passes
+    # carry no events, so that step is repeated eagerly (same kernels, same
+    # launch shape)
     first = steps[0]
-    achieved = alg_bytes / (first["extend_ms"] * 1e-3) / 1e9
+    if graph:
+        graph = False
+        first = step(args.warmup)
+        graph = True
+    if first["tail_ms"] > first["extend_ms"]:
+        # the fused tail carries the pass: its traversal bytes plus the path
+        # queue read (48 B) and radiance write (16 B) of every path
+        kernel, launches, k_ms = "k_tail", max(1, first["tail_launches"]), first["tail_ms"]
+        alg_bytes = alg_trav + 64 * first["paths"]
+    else:
+        kernel, launches, k_ms = "k_extend", max(1, first["extend_launches"]), first["extend_ms"]
+        alg_bytes = alg_trav
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    traffic = PMC_TRAFFIC.get((kernel, W, shape.get("tail_paths")))
     out = {
         "metric": METRIC,
         "value": paths / elapsed,
@@ -349,15 +402,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"cornell_box_{W}x{H}_depth50", "width": W, "height": H, "spp_per_step": S,
-                   "max_depth": 50, "seed": 0, "n_quads": int(arr.n_prims), "bvh_nodes": bvh["n_nodes"]},
+                   "max_depth": 50, "seed": 0, "n_quads": int(arr.n_prims), "bvh_nodes": bvh["n_nodes"],
+                   "launch": shape},
         "rays_per_s": segments / elapsed,
         "segments_per_path": segments / paths,
         "extend_ms_per_step": ext_ms / args.steps,
-        "extend_share": ext_ms / (elapsed * 1e3),
+        "tail_ms_per_step": tail_ms / args.steps,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_extend", "alg_bytes_per_launch": alg_bytes / max(1, first["extend_launches"]),
-                     "avg_launch_ms": first["extend_ms"] / max(1, first["extend_launches"])},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kernel, "alg_bytes_per_launch": alg_bytes / launches,
+                     "avg_launch_ms": k_ms / launches},
+        "autotune_table": tune_table,
     }
     if not args.no_metric:
         out["time_to_relmse"] = time_to_relmse_section(ctx, scene, cam)

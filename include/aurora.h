@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define AURORA_ABI_VERSION 1
+#define AURORA_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------- */
 #define AURORA_OK 0
@@ -147,7 +147,10 @@ typedef struct aurora_camera_desc {
 #define AURORA_RENDER_AOV 1u   /* also write first-hit albedo and normal */
 #define AURORA_RENDER_COUNT_FETCHES 2u  /* count BVH node and primitive
                                            records the traversal fetches
-                                           (slower; for roofline accounting) */
+                                           (slower; for roofline accounting;
+                                           every segment then runs in the
+                                           wavefront kernels, none in the
+                                           fused tail) */
 #define AURORA_RENDER_GRAPH 4u  /* capture one pass in a hipGraph and replay
                                    it for every pass of the same shape
                                    (fixed launch shapes; same results) */
@@ -171,10 +174,11 @@ typedef struct aurora_render_stats {
     uint32_t passes;
     uint32_t max_live;       /* largest wavefront queue                      */
     uint32_t extend_launches;
-    uint32_t reserved;
+    uint32_t tail_launches;  /* fused-tail launches (eager passes)           */
     uint64_t nodes_fetched;  /* AURORA_RENDER_COUNT_FETCHES: 32-byte nodes   */
     uint64_t spheres_fetched;/*   16-byte sphere records                     */
     uint64_t quads_fetched;  /*   48-byte quad records                       */
+    double tail_ms;          /* summed device time of the fused-tail launches */
 } aurora_render_stats;
 
 /* ---- context ---------------------------------------------------------- */
@@ -196,9 +200,11 @@ int aurora_synchronize(aurora_ctx* ctx);
 void* aurora_stream(aurora_ctx* ctx);
 /* Launch shape: persistent grid sizes of the traversal and shade kernels in
  * blocks per compute unit (default 8 each; both in [1, 64]), and the live
- * path count at or below which the remaining bounces of a pass run fused in
- * one kernel (default 1048576; 0 = never).  Changes speed only, never
- * results. */
+ * path count at or below which the remaining bounces of a pass run in the
+ * persistent fused-tail kernel, each lane carrying a path to termination
+ * and then taking the next (default 0xffffffff: the whole pass; 0 = never,
+ * the wavefront extend/shade kernels carry every bounce).  Changes speed
+ * only, never results. */
 int aurora_set_launch(aurora_ctx* ctx, int extend_blocks_per_cu,
                       int shade_blocks_per_cu, uint32_t tail_paths);
 

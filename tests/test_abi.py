@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert {e[0] for e in abi.EXPORTS} == set(names)
-    assert lib.aurora_abi_version() == 1
+    assert lib.aurora_abi_version() == 2
 
 
 def test_struct_sizes_match_c():

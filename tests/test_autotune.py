@@ -32,7 +32,7 @@ def test_picks_cheapest_and_caches(tmp_path, monkeypatch):
     best, table = autotune.tune(ctx, run, "sigA", cache_path=str(cache))
     assert (best["extend_blocks_per_cu"], best["tail_paths"], best["graph"]) == (8, 1 << 20, True)
     assert ctx.shape == (8, 8, 1 << 20)
-    n_cand = 3 * 1 * 3 * 2
+    n_cand = 3 * 1 * 2 * 2
     assert len(table) == n_cand and len(calls) == n_cand * 3  # warm-up + 2 repeats each
     saved = json.loads(cache.read_text())
     assert saved["sigA"]["best"]["graph"] is True

@@ -34,34 +34,51 @@ def workload(ctx):
     return items
 
 
+# fused-tail switch points: never (wavefront kernels only), part of the
+# pass, and always (one persistent kernel carries every path)
+TAILS = [0, 4096, (1 << 32) - 1]
+
+
 def test_batch_equals_single_renders(ctx):
     print("batch vs single", flush=True)
     items = workload(ctx)
     singles, seg = [], 0
-    for sc, cam, rs in items:
-        rgb, alb, nrm = zeros(cam), zeros(cam), zeros(cam)
-        st = sc.render_device(cam, rs, rgb, alb if rs.aov else None, nrm if rs.aov else None)
-        singles.append((rgb.cpu().numpy(), alb.cpu().numpy() if rs.aov else None))
-        seg += st["segments"]
-    for graph in (False, True):
-        bufs = [(zeros(cam), zeros(cam), zeros(cam)) for _, cam, _ in items]
-        st = render_batch(ctx, [(sc, cam, rs, b[0], b[1] if rs.aov else None, b[2] if rs.aov else None)
-                                for (sc, cam, rs), b in zip(items, bufs)], graph=graph)
-        print("graph", graph, "paths", st["paths"], "segments", st["segments"], "single", seg, flush=True)
-        assert st["segments"] == seg
-        for k, ((rgb, alb, _), (s_rgb, s_alb)) in enumerate(zip(bufs, singles)):
-            assert np.array_equal(rgb.cpu().numpy(), s_rgb), k
-            if s_alb is not None:
-                assert np.array_equal(alb.cpu().numpy(), s_alb), k
+    ctx.set_launch(tail_paths=0)
+    try:
+        for sc, cam, rs in items:
+            rgb, alb, nrm = zeros(cam), zeros(cam), zeros(cam)
+            st = sc.render_device(cam, rs, rgb, alb if rs.aov else None, nrm if rs.aov else None)
+            singles.append((rgb.cpu().numpy(), alb.cpu().numpy() if rs.aov else None))
+            seg += st["segments"]
+        for tail in TAILS:
+            ctx.set_launch(tail_paths=tail)
+            for graph in (False, True):
+                bufs = [(zeros(cam), zeros(cam), zeros(cam)) for _, cam, _ in items]
+                st = render_batch(ctx, [(sc, cam, rs, b[0], b[1] if rs.aov else None, b[2] if rs.aov else None)
+                                        for (sc, cam, rs), b in zip(items, bufs)], graph=graph)
+                print("tail", tail, "graph", graph, "paths", st["paths"], "segments", st["segments"],
+                      "single", seg, flush=True)
+                assert st["segments"] == seg
+                for k, ((rgb, alb, _), (s_rgb, s_alb)) in enumerate(zip(bufs, singles)):
+                    assert np.array_equal(rgb.cpu().numpy(), s_rgb), (tail, graph, k)
+                    if s_alb is not None:
+                        assert np.array_equal(alb.cpu().numpy(), s_alb), (tail, graph, k)
+    finally:
+        ctx.set_launch()
 
 
-def test_graph_render_equals_eager(ctx):
-    print("graph vs eager", flush=True)
+@pytest.mark.parametrize("tail", TAILS)
+def test_graph_render_equals_eager(ctx, tail):
+    print("graph vs eager, tail", tail, flush=True)
     sc = ctx.upload(scenes.cornell_box())
     sc.build_bvh()
     cam = scenes.cornell_camera(64)
     a, b = zeros(cam), zeros(cam)
-    sc.render_device(cam, RenderSettings(spp=12, spp_per_pass=2), a)
-    st = sc.render_device(cam, RenderSettings(spp=12, spp_per_pass=2, graph=True), b)
+    ctx.set_launch(tail_paths=tail)
+    try:
+        sc.render_device(cam, RenderSettings(spp=12, spp_per_pass=2), a)
+        st = sc.render_device(cam, RenderSettings(spp=12, spp_per_pass=2, graph=True), b)
+    finally:
+        ctx.set_launch()
     assert st["passes"] == 6
     assert torch.equal(a, b)
