@@ -22,7 +22,7 @@ import math
 import numpy as np
 
 from . import abi
-from .core import Camera, SceneArrays, checker, noise, solid
+from .core import Camera, SceneArrays, checker, image, noise, solid
 
 STREAM_RENDER = 0
 STREAM_SCENE = 1
@@ -260,6 +260,37 @@ def stress_scene(n_spheres=2_000_000, n_tris=2_000_000, seed=0):
     arr.quad_mat = mats[n_spheres:]
     arr.quad_kind = np.full(n_tris, abi.QUAD_TRIANGLE, np.uint32)
     return arr
+
+
+def textured_scene(seed=0):
+    """Every texture and material kind in one small scene: image textures
+    (texels drawn from Philox) on a sphere (spherical u, v), a parallelogram
+    and a triangle (barycentric u, v) and on an emitter; checker and Perlin
+    spheres; fuzzed metal; glass; a lambertian ground under a sky."""
+    rnd = Draws(seed)
+    b = Builder()
+    t1 = rnd.array(3 * 64 * 32).reshape(-1, 3)          # 64 x 32 image
+    t2 = rnd.array(3 * 16 * 16).reshape(-1, 3) * 4.0    # 16 x 16 emitter image
+    img1 = b.texture(image(64, 32, 0))
+    img2 = b.texture(image(16, 16, 64 * 32))
+    chk = b.texture(checker(0.3, b.texture(solid(0.9, 0.2, 0.1)), b.texture(solid(0.1, 0.1, 0.8))))
+    per = b.texture(noise(3.0, seed=seed))
+    m_img = b.material(abi.MAT_LAMBERTIAN, img1)
+    b.quad((-4, 0, -4), (8, 0, 0), (0, 0, 8), m_img)
+    b.sphere((0, 1, 0), 1.0, m_img)
+    b.sphere((-2.2, 0.8, 0.5), 0.8, b.material(abi.MAT_LAMBERTIAN, chk))
+    b.sphere((2.2, 0.8, 0.5), 0.8, b.material(abi.MAT_LAMBERTIAN, per))
+    b.sphere((1.0, 0.4, 2.0), 0.4, b.metal((0.8, 0.7, 0.6), 0.3))
+    b.sphere((-1.0, 0.4, 2.0), 0.4, b.dielectric(1.5))
+    b.quad((-3, 0.2, -2), (2.0, 2.5, 0.0), (1.5, 0.0, -0.5), m_img, kind=abi.QUAD_TRIANGLE)
+    b.quad((-1.5, 3.5, -1.0), (3.0, 0.0, 0.0), (0.0, 0.0, 2.0), b.material(abi.MAT_DIFFUSE_LIGHT, img2))
+    arr = b.arrays()
+    arr.texels = np.concatenate([t1, t2]).astype(np.float32)
+    return arr
+
+
+def textured_camera(width=64, height=48):
+    return Camera(width, height, 40.0, (0.0, 2.5, 7.0), (0.0, 0.8, 0.0), (0.0, 1.0, 0.0), 0.0, 7.0)
 
 
 def stress_camera(width=1920, height=1080):

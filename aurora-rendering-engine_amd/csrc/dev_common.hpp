@@ -158,6 +158,19 @@ __device__ __forceinline__ double dacos(double y) { return datan2(sqrt((1.0 - y)
 // ----------------------------------------------------------- scene view ---
 // Device texture record: aurora_texture with `even` reused as the Perlin
 // table slot of a noise texture.
+// Loads through the global address space.  Scene arrays reach the kernels
+// through tables in device memory, so the compiler cannot tell their
+// pointers from LDS pointers and would emit flat loads, which are slower to
+// issue and make every wait also wait for the LDS stack traffic.
+typedef float aur_f4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const aur_f4v* aur_g4p;
+typedef __attribute__((address_space(1))) const uint32_t* aur_g1p;
+__device__ __forceinline__ float4 ldg4(const float4* p, size_t i) {
+    const aur_f4v v = ((aur_g4p)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t ldg1(const uint32_t* p, size_t i) { return ((aur_g1p)p)[i]; }
+
 struct DevScene {
     const float4* spheres;        // (centre, radius)
     const uint32_t* sphere_mat;

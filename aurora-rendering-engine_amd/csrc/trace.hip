@@ -76,12 +76,12 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
         if (id >= n_prims) return;
         if (id < n_spheres) {
             if (COUNT) ++cs;
-            h = hit_sphere(S[id], o, d, tmin, tmax, t);
+            h = hit_sphere(ldg4(S, id), o, d, tmin, tmax, t);
         } else {
             if (COUNT) ++cq;
-            const float4* q = Qd + 3 * (size_t)(id - n_spheres);
+            const size_t q = 3 * (size_t)(id - n_spheres);
             float al, be;
-            h = hit_quad(q[0], q[1], q[2], o, d, tmin, tmax, t, al, be);
+            h = hit_quad(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t, al, be);
         }
         if (h && (t < best_t || (t == best_t && id < best_id))) {
             best_t = t;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
                     if (n_nodes == 0) {
                         finish();
                     } else {
-                        const float4 r0 = N[0], r1 = N[1];
+                        const float4 r0 = ldg4(N, 0), r1 = ldg4(N, 1);
                         if (COUNT) ++cn;
                         float tr;
                         const uint32_t rc = __float_as_uint(r0.w);
@@ -188,8 +188,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
             continue;
         }
         // one traversal step: the child pair at nodes[cur], nodes[cur+1]
-        const float4 a0 = N[2 * (size_t)cur], a1 = N[2 * (size_t)cur + 1];
-        const float4 b0 = N[2 * (size_t)cur + 2], b1 = N[2 * (size_t)cur + 3];
+        const float4 a0 = ldg4(N, 2 * (size_t)cur), a1 = ldg4(N, 2 * (size_t)cur + 1);
+        const float4 b0 = ldg4(N, 2 * (size_t)cur + 2), b1 = ldg4(N, 2 * (size_t)cur + 3);
         if (COUNT) cn += 2;
         float ta, tb;
         bool ha = slab(a0, a1, o, inv, tmin, best_t, ta);
@@ -220,7 +220,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
             finish();
         } else {
             --sp;
-            cur = sp < kLds ? stk[sp][tid] : spill[(size_t)(sp - kLds) * sstride];
+            if (sp < kLds) cur = stk[sp][tid];
+            else cur = ldg1(spill, (size_t)(sp - kLds) * sstride);
         }
     }
     if (COUNT && a.totals) {

@@ -33,11 +33,11 @@ __device__ __forceinline__ void traverse_closest(const DevScene& sc, v3 o, v3 d,
         float t;
         bool h;
         if (id < n_spheres) {
-            h = hit_sphere(S[id], o, d, tmin, tmax, t);
+            h = hit_sphere(ldg4(S, id), o, d, tmin, tmax, t);
         } else {
-            const float4* q = Qd + 3 * (size_t)(id - n_spheres);
+            const size_t q = 3 * (size_t)(id - n_spheres);
             float al, be;
-            h = hit_quad(q[0], q[1], q[2], o, d, tmin, tmax, t, al, be);
+            h = hit_quad(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t, al, be);
         }
         if (h && (t < best_t || (t == best_t && id < best_id))) {
             best_t = t;
@@ -45,7 +45,7 @@ __device__ __forceinline__ void traverse_closest(const DevScene& sc, v3 o, v3 d,
         }
     };
     if (n_nodes > 0) {
-        const float4 r0 = N[0], r1 = N[1];
+        const float4 r0 = ldg4(N, 0), r1 = ldg4(N, 1);
         float tr;
         const uint32_t rc = __float_as_uint(r0.w);
         if (slab(r0, r1, o, inv, tmin, best_t, tr)) {
@@ -56,8 +56,8 @@ __device__ __forceinline__ void traverse_closest(const DevScene& sc, v3 o, v3 d,
                 int sp = 0;
                 for (;;) {
                     if ((cur & 1u) || cur + 1 >= n_nodes) break;
-                    const float4 a0 = N[2 * (size_t)cur], a1 = N[2 * (size_t)cur + 1];
-                    const float4 b0 = N[2 * (size_t)cur + 2], b1 = N[2 * (size_t)cur + 3];
+                    const float4 a0 = ldg4(N, 2 * (size_t)cur), a1 = ldg4(N, 2 * (size_t)cur + 1);
+                    const float4 b0 = ldg4(N, 2 * (size_t)cur + 2), b1 = ldg4(N, 2 * (size_t)cur + 3);
                     float ta, tb;
                     bool ha = slab(a0, a1, o, inv, tmin, best_t, ta);
                     bool hb = slab(b0, b1, o, inv, tmin, best_t, tb);
@@ -85,7 +85,8 @@ __device__ __forceinline__ void traverse_closest(const DevScene& sc, v3 o, v3 d,
                         break;
                     } else {
                         --sp;
-                        cur = sp < kTravLds ? stk[sp * col_stride] : spill[(size_t)(sp - kTravLds) * sstride];
+                        if (sp < kTravLds) cur = stk[sp * col_stride];
+                        else cur = ldg1(spill, (size_t)(sp - kTravLds) * sstride);
                     }
                 }
             }

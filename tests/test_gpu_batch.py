@@ -23,6 +23,8 @@ def workload(ctx):
         (scenes.stress_scene(3000, 3000, seed=9), scenes.stress_camera(32, 18),
          RenderSettings(spp=2, max_depth=16, background=(0.7, 0.8, 1.0), tmin=1e-4)),
     ]
+    specs.append((scenes.textured_scene(1), scenes.textured_camera(32, 24),
+                  RenderSettings(spp=2, max_depth=12, background=(0.6, 0.7, 0.9), aov=True)))
     for seed in (1, 2, 3):
         arr, cam, spp = scenes.batch_scene(seed, max_prims=5000)
         cam.image_width = cam.image_height = 32

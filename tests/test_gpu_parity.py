@@ -144,6 +144,31 @@ def test_render_radiance_bit_exact(ctx, name, mk, mkcam, rs):
         assert eq.all()
 
 
+def test_render_textures_bit_exact(ctx):
+    """Image textures on a sphere (spherical u, v from the series atan2 and
+    acos), a parallelogram, a triangle and an emitter, with checker, Perlin,
+    fuzzed metal and glass: radiance and AOVs bit-exact against the oracle
+    in the wavefront-only and persistent-tail paths."""
+    say("render textures")
+    arr = scenes.textured_scene(0)
+    cam = scenes.textured_camera(64, 48)
+    g, gs, o, os_ = both(ctx, arr)
+    rs = RenderSettings(spp=4, max_depth=20, background=(0.6, 0.7, 0.9), aov=True)
+    orgb, oalb, onrm, segs = o.render(cam, rs)
+    assert len(np.unique(oalb.reshape(-1, 3), axis=0)) > 500  # the textures vary across the image
+    for tail in (0, (1 << 32) - 1):
+        ctx.set_launch(tail_paths=tail)
+        try:
+            grgb, galb, gnrm, st = g.render(cam, rs)
+        finally:
+            ctx.set_launch()
+        say("textures tail", tail, "segments gpu/cpu", st["segments"], segs)
+        assert st["segments"] == segs
+        assert np.array_equal(grgb.view(np.uint32), orgb.view(np.uint32))
+        assert np.array_equal(galb.view(np.uint32), oalb.view(np.uint32))
+        assert np.array_equal(gnrm.view(np.uint32), onrm.view(np.uint32))
+
+
 def test_render_aov_and_progressive(ctx):
     say("render aov/progressive")
     arr = scenes.cornell_box()
