@@ -43,6 +43,8 @@ int orc_hit_sphere(const float* sphere, const float* o, const float* d, float tm
 int orc_hit_quad(const float* quad, uint32_t kind, const float* o, const float* d, float tmin, float tmax,
                  float* tab);
 void orc_texture_value(orc_scene* s, uint32_t tex, float u, float v, const float* p, float* out);
+/* (u, v) of the outward unit normal n on a sphere, as shading computes it */
+void orc_sphere_uv(const float* n, float* uv);
 /* out18: center, pixel00, delta_u, delta_v, disk_u, disk_v */
 int orc_camera(const aurora_camera_desc* c, float* out18);
 /* pixels NULL = every pixel in row-major order; rgb/albedo/normal are 3 per

@@ -45,6 +45,7 @@ def lib():
             "orc_hit_sphere": (C.c_int, [vp, vp, vp, f, f, vp]),
             "orc_hit_quad": (C.c_int, [vp, u32, vp, vp, f, f, vp]),
             "orc_texture_value": (None, [vp, u32, f, f, vp, vp]),
+            "orc_sphere_uv": (None, [vp, vp]),
             "orc_camera": (C.c_int, [C.POINTER(abi.CameraDesc), vp]),
             "orc_render_pixels": (C.c_int, [vp, C.POINTER(abi.CameraDesc), C.POINTER(abi.RenderDesc), vp, u32,
                                             C.c_int, vp, vp, vp, vp]),
@@ -83,6 +84,13 @@ def perlin(seed):
     p = np.zeros((3, 256), np.int32)
     lib().orc_perlin(seed, v.ctypes.data, p.ctypes.data)
     return v, p
+
+
+def sphere_uv(n):
+    nn = np.asarray(n, np.float32)
+    uv = np.zeros(2, np.float32)
+    lib().orc_sphere_uv(nn.ctypes.data, uv.ctypes.data)
+    return uv
 
 
 def reflect(v, n):

@@ -403,6 +403,13 @@ int orc_hit_quad(const float* quad, uint32_t kind, const float* o, const float* 
                                                                                                                 : 0;
 }
 
+void orc_sphere_uv(const float* n, float* uv) {
+    float u, v;
+    sphere_uv<float>(mk(n[0], n[1], n[2]), u, v);
+    uv[0] = u;
+    uv[1] = v;
+}
+
 void orc_texture_value(orc_scene* sc, uint32_t tex, float u, float v, const float* p, float* out) {
     V3<float> c = texture_value<float>(sc->s, tex, u, v, mk(p[0], p[1], p[2]));
     out[0] = c.x; out[1] = c.y; out[2] = c.z;
