@@ -20,6 +20,7 @@ MAT_METAL = 1
 MAT_DIELECTRIC = 2
 MAT_DIFFUSE_LIGHT = 3
 
+BVH4_EMPTY = 0xFFFFFFFF
 TEX_SOLID = 0
 TEX_CHECKER = 1
 TEX_NOISE = 2
@@ -80,7 +81,7 @@ class BvhStats(C.Structure):
         ("n_prims", C.c_uint32),
         ("n_nodes", C.c_uint32),
         ("depth", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("n_wide", C.c_uint32),
         ("scene_lo", C.c_float * 3),
         ("scene_hi", C.c_float * 3),
         ("centroid_lo", C.c_float * 3),
@@ -162,6 +163,7 @@ EXPORTS = [
     ("aurora_scene_destroy", None, [_vp]),
     ("aurora_bvh_build", C.c_int, [_vp, C.POINTER(BvhStats)]),
     ("aurora_bvh_download", C.c_int, [_vp, _vp, _u32]),
+    ("aurora_bvh_download_wide", C.c_int, [_vp, _vp, _u32]),
     ("aurora_bvh_download_order", C.c_int, [_vp, _vp, _vp, _u32]),
     ("aurora_trace_closest", C.c_int, [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp]),
     ("aurora_trace_closest_device", C.c_int, [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp, _vp]),

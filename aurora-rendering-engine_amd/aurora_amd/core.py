@@ -230,6 +230,13 @@ class Scene:
         self.ctx.check(self.lib.aurora_bvh_download(self.h, buf.ctypes.data, n))
         return buf[:n]
 
+    def wide(self):
+        """Four-wide traversal nodes as (n_wide, 32) uint32 words."""
+        n = self.bvh_stats["n_wide"]
+        buf = np.zeros((max(n, 1), 32), np.uint32)
+        self.ctx.check(self.lib.aurora_bvh_download_wide(self.h, buf.ctypes.data, n))
+        return buf[:n]
+
     def order(self):
         n = self.arrays.n_prims
         m = np.zeros(max(n, 1), np.uint32)

@@ -48,7 +48,7 @@ struct aurora_scene {
     DevScene view{};
     uint32_t n_prims = 0;
     DevBuf spheres, sphere_mat, quads, mats, texs, texels, perlin_vec, perlin_perm;
-    DevBuf nodes, morton, order;
+    DevBuf nodes, wide, morton, order;
     BvhScratch scr;
     bool built = false;
     aurora_bvh_stats stats{};
@@ -291,6 +291,7 @@ Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n,
         a.spill_stride = (uint32_t)(grid * extend_block());
         a.recs = s->rec1.as<SceneRec>();
         a.one_scene = 1;
+        a.wide = 1;  // ray queries: memory bound, the four-wide tree wins
         // Large queries on large trees: claim rays in (origin Morton code,
         // octant) order, so the rays in flight on one XCD share the nodes
         // its L2 holds.  The hit of each ray is unchanged.
@@ -836,10 +837,11 @@ void aurora_scene_destroy(aurora_scene* s) {
     if (!s) return;
     if (s->ctx && s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
     DevBuf* bufs[] = {&s->spheres, &s->sphere_mat, &s->quads, &s->mats, &s->texs, &s->texels, &s->perlin_vec,
-                      &s->perlin_perm, &s->nodes, &s->morton, &s->order, &s->scr.box_lo, &s->scr.box_hi,
+                      &s->perlin_perm, &s->nodes, &s->wide, &s->morton, &s->order, &s->scr.box_lo, &s->scr.box_hi,
                       &s->scr.bounds, &s->scr.keys[0], &s->scr.keys[1], &s->scr.vals[0], &s->scr.vals[1],
                       &s->scr.hist, &s->scr.gamma, &s->scr.flags, &s->scr.int_parent, &s->scr.leaf_parent,
-                      &s->scr.int_pos, &s->scr.leaf_pos, &s->scr.arrive, &s->scr.depth, &s->rec1, &s->tr_o, &s->tr_d,
+                      &s->scr.int_pos, &s->scr.leaf_pos, &s->scr.arrive, &s->scr.depth, &s->scr.widx, &s->rec1,
+                      &s->tr_o, &s->tr_d,
                       &s->tr_hit, &s->tr_spill, &s->tr_pool, &s->totals, &s->tr_key[0], &s->tr_key[1],
                       &s->tr_val[0], &s->tr_val[1], &s->tr_hist};
     for (DevBuf* b : bufs) b->release();
@@ -851,9 +853,11 @@ int aurora_bvh_build(aurora_scene* s, aurora_bvh_stats* stats) {
     aurora_ctx* c = s->ctx;
     CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
     aurora_bvh_stats st{};
-    CK(c, lbvh_build(s->view, s->n_prims, s->nodes, s->morton, s->order, s->scr, c->stream, &st));
+    CK(c, lbvh_build(s->view, s->n_prims, s->nodes, s->wide, s->morton, s->order, s->scr, c->stream, &st));
     s->view.nodes = s->nodes.as<float4>();
     s->view.n_nodes = st.n_nodes;
+    s->view.wide = s->wide.as<float4>();
+    s->view.n_wide = st.n_nodes ? st.n_wide : 0u;
     s->stats = st;
     s->built = true;
     if (stats) *stats = st;
@@ -869,6 +873,20 @@ int aurora_bvh_download(aurora_scene* s, aurora_bvh_node* nodes, uint32_t capaci
     if (!nodes) return fail(c, AURORA_ERR_INVALID, "nodes is NULL");
     CK(c, hip_status(hipMemcpyAsync(nodes, s->nodes.p, (size_t)s->stats.n_nodes * 32, hipMemcpyDeviceToHost,
                                     c->stream),
+                     "hipMemcpyAsync"));
+    CK(c, hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
+    return AURORA_OK;
+}
+
+int aurora_bvh_download_wide(aurora_scene* s, aurora_bvh4_node* nodes, uint32_t capacity) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    aurora_ctx* c = s->ctx;
+    if (!s->built) return fail(c, AURORA_ERR_STATE, "BVH not built");
+    const uint32_t n = s->view.n_wide;
+    if (capacity < n) return fail(c, AURORA_ERR_INVALID, "capacity smaller than the wide node count");
+    if (n == 0) return AURORA_OK;
+    if (!nodes) return fail(c, AURORA_ERR_INVALID, "nodes is NULL");
+    CK(c, hip_status(hipMemcpyAsync(nodes, s->wide.p, (size_t)n * 128, hipMemcpyDeviceToHost, c->stream),
                      "hipMemcpyAsync"));
     CK(c, hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
     return AURORA_OK;

@@ -181,7 +181,8 @@ struct DevScene {
     const float* perlin_vec;      // per slot: 256 x 3 floats
     const int* perlin_perm;       // per slot: 3 x 256 ints
     const float4* nodes;          // 2 float4 per node: (lo, child) (hi, parent)
-    uint32_t n_spheres, n_quads, n_nodes, pad;
+    const float4* wide;           // four-wide traversal nodes: 4 slots of (lo, ref) (hi, 0)
+    uint32_t n_spheres, n_quads, n_nodes, n_wide;
 };
 
 __device__ __forceinline__ bool hit_sphere(float4 sp, v3 o, v3 d, float tmin, float tmax, float& t) {

@@ -68,11 +68,13 @@ struct BvhScratch {
     DevBuf keys[2], vals[2];    // radix sort ping-pong
     DevBuf hist;                // 256 * blocks
     DevBuf gamma, flags, int_parent, leaf_parent, int_pos, leaf_pos, arrive, depth;
+    DevBuf widx;                // wide index of each internal node (exclusive scan of even-depth flags)
 };
 
-// Build the LBVH into `nodes` (2n float4 pairs) on `stream`.  morton/order
-// receive the sorted codes and primitive order.
-Status lbvh_build(const DevScene& sc, uint32_t n_prims, DevBuf& nodes, DevBuf& morton, DevBuf& order,
+// Build the LBVH into `nodes` (2n float4 pairs) and the four-wide traversal
+// tree collapsed from it into `wide` (8 float4 per node) on `stream`.
+// morton/order receive the sorted codes and primitive order.
+Status lbvh_build(const DevScene& sc, uint32_t n_prims, DevBuf& nodes, DevBuf& wide, DevBuf& morton, DevBuf& order,
                   BvhScratch& scr, hipStream_t stream, aurora_bvh_stats* stats);
 
 // Stable LSD radix sort of (key, value) pairs, 8 bits per pass from bit 0.
@@ -94,6 +96,7 @@ struct ExtendArgs {
     float2* hit;              // (t, prim bits)
     const struct SceneRec* recs;  // scene table; a ray's scene is the bits of ray_d.w
     uint32_t one_scene;       // nonzero: every ray belongs to recs[0]
+    uint32_t wide;            // nonzero: walk the four-wide tree, else the binary one
     const uint32_t* order;    // optional: claim order of the rays (slot -> ray index)
     uint32_t* counts;         // optional: 3 per ray (nodes, spheres, quads fetched)
     unsigned long long* totals; // optional: [nodes, spheres, quads] summed

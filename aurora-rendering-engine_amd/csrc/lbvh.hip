@@ -415,6 +415,72 @@ __global__ __launch_bounds__(kBlock) void k_depth(uint32_t n, const uint32_t* le
     atomicMax(depth, d);
 }
 
+// ---- four-wide traversal tree ----
+// flag[i] = 1 when internal node i has even depth (it becomes a wide node);
+// flag[ni] = 0, so the exclusive scan leaves the wide-node count at [ni]
+__global__ __launch_bounds__(kBlock) void k_wide_flags(uint32_t ni, const uint32_t* int_parent, uint32_t* f) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i > ni) return;
+    if (i == ni) {
+        f[i] = 0u;
+        return;
+    }
+    uint32_t d = 0, p = i;
+    while (p != 0 && p < ni && d < 256) {
+        p = int_parent[p];
+        ++d;
+    }
+    f[i] = (d & 1u) == 0 ? 1u : 0u;
+}
+
+__device__ __forceinline__ void wide_slot(float4* w, int k, const float4* nodes, uint32_t pos, uint32_t ref) {
+    const float4 lo = nodes[2 * (size_t)pos], hi = nodes[2 * (size_t)pos + 1];
+    w[2 * k] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(ref));
+    w[2 * k + 1] = make_float4(hi.x, hi.y, hi.z, 0.0f);
+}
+
+// one wide node per even-depth internal node i, at widx[i]: the expansions
+// of its two children in order (a leaf child is itself, an internal child
+// contributes its two children); unused slots are empty
+__global__ __launch_bounds__(kBlock) void k_wide_fill(uint32_t ni, const uint32_t* gamma, const uint32_t* flags,
+                                                      const uint32_t* widx, const float4* nodes, float4* wide) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ni || widx[i + 1] == widx[i]) return;
+    float4* w = wide + 8 * (size_t)widx[i];
+    const uint32_t g = gamma[i], fl = flags[i];
+    int k = 0;
+    for (uint32_t c = 0; c < 2; ++c) {
+        const uint32_t pos = 2 + 2 * g + c;
+        if (fl & (1u << c)) {
+            wide_slot(w, k++, nodes, pos, __float_as_uint(nodes[2 * (size_t)pos].w));
+            continue;
+        }
+        const uint32_t j = g + c;
+        const uint32_t gj = gamma[j], flj = flags[j];
+        for (uint32_t e = 0; e < 2; ++e) {
+            const uint32_t gpos = 2 + 2 * gj + e;
+            const uint32_t ref = (flj & (1u << e)) ? __float_as_uint(nodes[2 * (size_t)gpos].w) : widx[gj + e];
+            wide_slot(w, k++, nodes, gpos, ref);
+        }
+    }
+    for (; k < 4; ++k) {
+        w[2 * k] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(AURORA_BVH4_EMPTY));
+        w[2 * k + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// a single primitive: one wide node holding the leaf
+__global__ void k_wide_single(const float4* nodes, float4* wide) {
+    const int k = threadIdx.x;
+    if (k >= 4) return;
+    if (k == 0) {
+        wide_slot(wide, 0, nodes, 0, __float_as_uint(nodes[0].w));
+    } else {
+        wide[2 * k] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(AURORA_BVH4_EMPTY));
+        wide[2 * k + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 // histogram entries of a sort of n keys, and the tile sums of its scan after them
@@ -451,7 +517,7 @@ Status radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, 
     return Status::ok();
 }
 
-Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& morton, DevBuf& order,
+Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide, DevBuf& morton, DevBuf& order,
                   BvhScratch& s, hipStream_t st, aurora_bvh_stats* stats) {
     DevScene sc = sc_in;
     aurora_bvh_stats local{};
@@ -480,6 +546,11 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& mort
     AUR_TRY(s.leaf_parent.alloc((size_t)n * 4));
     AUR_TRY(s.leaf_pos.alloc((size_t)n * 4));
     AUR_TRY(s.depth.alloc(4));
+    const uint32_t n_flags = (uint32_t)ni + 1;
+    AUR_TRY(s.widx.alloc(((size_t)n_flags + blocks_for(n_flags, kScanTile)) * 4));
+    // the wide-node count is known only after the scan: room for its bound
+    // (every internal node)
+    AUR_TRY(wide.alloc(ni * 128));
 
     hipEvent_t e0, e1;
     AUR_HIP(hipEventCreate(&e0));
@@ -512,6 +583,19 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& mort
     k_fill<<<1, kBlock, 0, st>>>(s.depth.as<uint32_t>(), 1, 0u);
     k_depth<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.leaf_parent.as<uint32_t>(), s.int_parent.as<uint32_t>(),
                                                       s.depth.as<uint32_t>());
+    // four-wide traversal tree
+    uint32_t n_wide = 1;
+    if (n > 1) {
+        const uint32_t nint = n - 1;
+        k_wide_flags<<<blocks_for(nint + 1, kBlock), kBlock, 0, st>>>(nint, s.int_parent.as<uint32_t>(),
+                                                                      s.widx.as<uint32_t>());
+        scan_exclusive(s.widx.as<uint32_t>(), nint + 1, st);
+        k_wide_fill<<<blocks_for(nint, kBlock), kBlock, 0, st>>>(nint, s.gamma.as<uint32_t>(), s.flags.as<uint32_t>(),
+                                                                 s.widx.as<uint32_t>(), nodes.as<float4>(),
+                                                                 wide.as<float4>());
+    } else {
+        k_wide_single<<<1, 64, 0, st>>>(nodes.as<float4>(), wide.as<float4>());
+    }
     AUR_HIP(hipGetLastError());
     AUR_HIP(hipEventRecord(e1, st));
     std::swap(morton, s.keys[0]);
@@ -520,12 +604,14 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& mort
     uint32_t b[12], depth = 0;
     AUR_HIP(hipMemcpyAsync(b, s.bounds.p, 48, hipMemcpyDeviceToHost, st));
     AUR_HIP(hipMemcpyAsync(&depth, s.depth.p, 4, hipMemcpyDeviceToHost, st));
+    if (n > 1) AUR_HIP(hipMemcpyAsync(&n_wide, s.widx.as<uint32_t>() + (n - 1), 4, hipMemcpyDeviceToHost, st));
     AUR_HIP(hipStreamSynchronize(st));
     float ms = 0.0f;
     AUR_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     local.n_nodes = n_nodes;
+    local.n_wide = n_wide;
     local.depth = depth;
     for (int k = 0; k < 3; ++k) {
         local.scene_lo[k] = o2f(b[k]);

@@ -340,7 +340,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         ++segs;
         float t;
         int prim;
-        traverse_closest(r.sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill, spill_stride, t,
+        traverse_closest_binary(r.sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill, spill_stride, t,
                          prim);
         const ShadeOut o = shade_core(r, b, org, dir, beta, t, prim, path, depth);
         if (!o.alive) {

@@ -4,18 +4,19 @@
 // k_extend is persistent: a grid sized to the resident capacity of the chip
 // pulls rays from a global pool.  Each lane owns one ray; when enough lanes
 // of a wave64 have finished, the wave claims that many new rays with one
-// atomic and hands them out by ballot prefix (refilling idle lanes).  One
-// step fetches a 64-byte aligned child pair (both boxes), tests both slabs,
-// intersects leaf primitives at once and descends to the nearer child,
-// pushing the farther.  The stack lives in LDS (kLds entries per lane, laid
-// out [level][thread] so a wave's pushes hit 64 consecutive banks) and
-// spills deeper levels to a per-thread global array; the Karras tree depth
-// is below 64 because every level lengthens the 64-bit key prefix.
+// atomic and hands them out by ballot prefix (refilling idle lanes).  The
+// rays walk the four-wide tree collapsed from the LBVH: one step fetches a
+// 128-byte node (one cache line, four child boxes), tests the four slabs,
+// intersects leaf primitives at once and descends to the nearest hit child,
+// pushing the others farthest first.  The stack lives in LDS (kLds entries
+// per lane, laid out [level][thread] so a wave's pushes hit 64 consecutive
+// banks) and spills deeper levels to a per-thread global array.
 //
 // Reference counterpart: the per-ray linear scan over all triangles,
 // experiments/rt.cpp:209-218, with are::Triangle::intersect_ray
 // (src/object/triangle.cpp:82-121) as the primitive test.
 #include "internal.hpp"
+#include "traverse.hpp"
 
 namespace aur {
 
@@ -23,7 +24,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kLds = 16;
-constexpr int kMaxStack = 64;
+constexpr int kMaxStack = 96;  // three pushes per level of a tree of depth <= 64
 constexpr int kRefill = 8;
 constexpr int kParts = 8;
 constexpr uint32_t kNone = 0xffffffffu;
@@ -32,7 +33,10 @@ constexpr uint32_t kNoPrim = 0x7fffffffu;
 // ONE: every ray belongs to the first scene record, whose fields are then
 // wave-uniform (scalar registers, loaded once); otherwise each claimed ray
 // loads its scene's record.
-template <bool COUNT, bool ONE>
+//
+// WIDE: walk the four-wide tree (ray queries, memory bound); otherwise the
+// binary tree, one 64-byte child pair per step (render bounces, issue bound).
+template <bool COUNT, bool ONE, bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
     __shared__ uint32_t stk[kLds][kBlock];
     const uint32_t n = a.count ? *a.count : a.n;
@@ -45,18 +49,18 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
     const float tmax = a.tmax;
     float tmin = 0.0f;  // the scene's ray offset
     // the scene of the lane's current ray
-    const float4* N = nullptr;
+    const float4* N = nullptr;  // the wide nodes (WIDE) or the binary nodes
     const float4* S = nullptr;
     const float4* Qd = nullptr;
     uint32_t n_spheres = 0, n_nodes = 0, n_prims = 0;
     if constexpr (ONE) {
         const SceneRec& rec = a.recs[0];
         tmin = rec.tmin;
-        N = rec.sc.nodes;
+        N = WIDE ? rec.sc.wide : rec.sc.nodes;
         S = rec.sc.spheres;
         Qd = rec.sc.quads;
         n_spheres = rec.sc.n_spheres;
-        n_nodes = rec.sc.n_nodes;
+        n_nodes = WIDE ? rec.sc.n_wide : rec.sc.n_nodes;
         n_prims = rec.sc.n_spheres + rec.sc.n_quads;
     }
 
@@ -146,11 +150,11 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
                     if constexpr (!ONE) {
                         const SceneRec& rec = a.recs[__float_as_uint(rd.w)];
                         tmin = rec.tmin;
-                        N = rec.sc.nodes;
+                        N = WIDE ? rec.sc.wide : rec.sc.nodes;
                         S = rec.sc.spheres;
                         Qd = rec.sc.quads;
                         n_spheres = rec.sc.n_spheres;
-                        n_nodes = rec.sc.n_nodes;
+                        n_nodes = WIDE ? rec.sc.n_wide : rec.sc.n_nodes;
                         n_prims = rec.sc.n_spheres + rec.sc.n_quads;
                     }
                     inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -160,6 +164,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
                     cn = cs = cq = 0;
                     if (n_nodes == 0) {
                         finish();
+                    } else if constexpr (WIDE) {
+                        cur = 0;  // the root of the four-wide tree
                     } else {
                         const float4 r0 = ldg4(N, 0), r1 = ldg4(N, 1);
                         if (COUNT) ++cn;
@@ -181,47 +187,76 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
         if (__ballot((ri != kNone) || !done) == 0) break;
         if (ri == kNone) continue;
 
-        // a well-formed tree only holds even pair indices below n_nodes; any
-        // other value ends the ray instead of reading out of bounds
-        if ((cur & 1u) || cur + 1 >= n_nodes) {
-            finish();
-            continue;
-        }
-        // one traversal step: the child pair at nodes[cur], nodes[cur+1]
-        const float4 a0 = ldg4(N, 2 * (size_t)cur), a1 = ldg4(N, 2 * (size_t)cur + 1);
-        const float4 b0 = ldg4(N, 2 * (size_t)cur + 2), b1 = ldg4(N, 2 * (size_t)cur + 3);
-        if (COUNT) cn += 2;
-        float ta, tb;
-        bool ha = slab(a0, a1, o, inv, tmin, best_t, ta);
-        bool hb = slab(b0, b1, o, inv, tmin, best_t, tb);
-        const uint32_t ca = __float_as_uint(a0.w), cb = __float_as_uint(b0.w);
-        if (ha && (ca & AURORA_NODE_LEAF)) {
-            test_prim(ca & ~AURORA_NODE_LEAF);
-            ha = false;
-        }
-        if (hb && (cb & AURORA_NODE_LEAF)) {
-            test_prim(cb & ~AURORA_NODE_LEAF);
-            hb = false;
-        }
-        if (ha && hb) {
-            const bool bnear = tb < ta;
-            const uint32_t far_ = bnear ? ca : cb;
-            cur = bnear ? cb : ca;
-            // the tree is shallower than kMaxStack; the bound only keeps every
+        auto push = [&](uint32_t v) {
+            // the tree is shallow enough that the bound only keeps every
             // access inside the stack arrays
-            if (sp < kLds) stk[sp][tid] = far_;
-            else if (sp < kMaxStack) spill[(size_t)(sp - kLds) * sstride] = far_;
+            if (sp < kLds) stk[sp][tid] = v;
+            else if (sp < kMaxStack) spill[(size_t)(sp - kLds) * sstride] = v;
             if (sp < kMaxStack) ++sp;
-        } else if (ha) {
-            cur = ca;
-        } else if (hb) {
-            cur = cb;
-        } else if (sp == 0) {
-            finish();
+        };
+        auto pop_or_finish = [&]() {
+            if (sp == 0) {
+                finish();
+            } else {
+                --sp;
+                if (sp < kLds) cur = stk[sp][tid];
+                else cur = ldg1(spill, (size_t)(sp - kLds) * sstride);
+            }
+        };
+        if constexpr (WIDE) {
+            // a well-formed tree only refers to nodes below n_wide; any other
+            // value ends the ray instead of reading out of bounds
+            if (cur >= n_nodes) {
+                finish();
+                continue;
+            }
+            // one step: the four slots of wide node cur
+            uint32_t r0, r1, r2, r3;
+            const int m = wide_step(N, cur, o, inv, tmin, best_t, test_prim, r0, r1, r2, r3);
+            if (COUNT) cn += 4;
+            if (m == 0) {
+                pop_or_finish();
+                continue;
+            }
+            // the others farthest first
+            if (m > 3) push(r3);
+            if (m > 2) push(r2);
+            if (m > 1) push(r1);
+            cur = r0;
         } else {
-            --sp;
-            if (sp < kLds) cur = stk[sp][tid];
-            else cur = ldg1(spill, (size_t)(sp - kLds) * sstride);
+            // a well-formed tree only holds even pair indices below n_nodes;
+            // any other value ends the ray instead of reading out of bounds
+            if ((cur & 1u) || cur + 1 >= n_nodes) {
+                finish();
+                continue;
+            }
+            // one step: the child pair at nodes[cur], nodes[cur+1]
+            const float4 a0 = ldg4(N, 2 * (size_t)cur), a1 = ldg4(N, 2 * (size_t)cur + 1);
+            const float4 b0 = ldg4(N, 2 * (size_t)cur + 2), b1 = ldg4(N, 2 * (size_t)cur + 3);
+            if (COUNT) cn += 2;
+            float ta, tb;
+            bool ha = slab(a0, a1, o, inv, tmin, best_t, ta);
+            bool hb = slab(b0, b1, o, inv, tmin, best_t, tb);
+            const uint32_t ca = __float_as_uint(a0.w), cb = __float_as_uint(b0.w);
+            if (ha && (ca & AURORA_NODE_LEAF)) {
+                test_prim(ca & ~AURORA_NODE_LEAF);
+                ha = false;
+            }
+            if (hb && (cb & AURORA_NODE_LEAF)) {
+                test_prim(cb & ~AURORA_NODE_LEAF);
+                hb = false;
+            }
+            if (ha && hb) {
+                const bool bnear = tb < ta;
+                push(bnear ? ca : cb);
+                cur = bnear ? cb : ca;
+            } else if (ha) {
+                cur = ca;
+            } else if (hb) {
+                cur = cb;
+            } else {
+                pop_or_finish();
+            }
         }
     }
     if (COUNT && a.totals) {
@@ -311,10 +346,12 @@ int extend_spill_levels() { return kMaxStack - kLds; }
 Status launch_extend(const ExtendArgs& a, int grid, hipStream_t s) {
     if ((size_t)grid * kBlock > a.spill_stride) return Status{AURORA_ERR_RUNTIME, "extend grid exceeds spill stride"};
     const bool count = a.counts || a.totals;
-    if (count && a.one_scene) k_extend<true, true><<<grid, kBlock, 0, s>>>(a);
-    else if (count) k_extend<true, false><<<grid, kBlock, 0, s>>>(a);
-    else if (a.one_scene) k_extend<false, true><<<grid, kBlock, 0, s>>>(a);
-    else k_extend<false, false><<<grid, kBlock, 0, s>>>(a);
+    const int v = (count ? 4 : 0) | (a.one_scene ? 2 : 0) | (a.wide ? 1 : 0);
+    void (*const ks[8])(ExtendArgs) = {k_extend<false, false, false>, k_extend<false, false, true>,
+                                       k_extend<false, true, false>,  k_extend<false, true, true>,
+                                       k_extend<true, false, false>,  k_extend<true, false, true>,
+                                       k_extend<true, true, false>,   k_extend<true, true, true>};
+    ks[v]<<<grid, kBlock, 0, s>>>(a);
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }

@@ -1,9 +1,13 @@
-// traverse.hpp — single-ray LBVH closest hit, for kernels that trace a ray
-// to completion in one thread (the fused tail of the wavefront loop).  Same
-// algorithm and operation order as k_extend's step (trace.hip): root test,
-// then per step the aligned child pair, both slabs against the current best
-// t, leaf primitives tested at once, descend to the nearer child and push
-// the farther; ties in t go to the lower primitive id.
+// traverse.hpp — closest-hit traversal over the LBVH, in two forms.
+// Four-wide: the 128-byte node (one cache line) holds four child boxes; per
+// step the four slab tests against the current best t, the hit leaf
+// primitives in slot order, then descend to the nearest hit internal slot
+// and push the others farthest first.  It halves the dependent fetches and
+// uses whole lines, so it wins where traversal is memory bound (large
+// incoherent ray queries).  Binary: the 64-byte child pair per step, a
+// cheaper step that wins where traversal is issue bound (render bounces,
+// whose nodes sit in cache).  Ties in t go to the lower primitive id, so
+// the closest hit does not depend on the form or the order.
 #pragma once
 
 #include "dev_common.hpp"
@@ -11,11 +15,69 @@
 namespace aur {
 
 constexpr int kTravLds = 16;       // stack levels kept in LDS
-constexpr int kTravMaxStack = 64;  // Karras trees are shallower than this
+constexpr int kTravMaxStack = 96;  // a four-wide step pushes at most 3 per level (depth <= 64)
 
+// Sort key of a hit internal slot: entry distance (ordered bits; -0 taken
+// as +0 so that the order is the float order), then slot index.
+__device__ __forceinline__ uint64_t wide_key(float t, uint32_t slot) {
+    uint32_t u = __float_as_uint(t + 0.0f);
+    u ^= (u & 0x80000000u) ? 0xffffffffu : 0x80000000u;
+    return ((uint64_t)u << 2) | slot;
+}
+__device__ __forceinline__ void wide_cs(uint64_t& a, uint64_t& b) {
+    const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
+
+// One step of the four-wide traversal at node `cur` (8 float4: per slot
+// (lo, ref), (hi, 0)): slab tests of the four slots against tcur, then the
+// hit leaf slots' primitives in slot order (test_prim), then the hit
+// internal slots ordered by entry distance, ties by slot.  Returns their
+// count; ref[0..count) holds them nearest first.
+template <class TestPrim>
+__device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3 inv, float tmin, float tcur,
+                                         TestPrim&& test_prim, uint32_t& r0, uint32_t& r1, uint32_t& r2,
+                                         uint32_t& r3) {
+    const size_t b = 8 * (size_t)cur;
+    const float4 l0 = ldg4(W, b), h0 = ldg4(W, b + 1), l1 = ldg4(W, b + 2), h1 = ldg4(W, b + 3);
+    const float4 l2 = ldg4(W, b + 4), h2 = ldg4(W, b + 5), l3 = ldg4(W, b + 6), h3 = ldg4(W, b + 7);
+    const uint32_t f0 = __float_as_uint(l0.w), f1 = __float_as_uint(l1.w);
+    const uint32_t f2 = __float_as_uint(l2.w), f3 = __float_as_uint(l3.w);
+    float t0, t1, t2, t3;
+    bool e0 = f0 != AURORA_BVH4_EMPTY && slab(l0, h0, o, inv, tmin, tcur, t0);
+    bool e1 = f1 != AURORA_BVH4_EMPTY && slab(l1, h1, o, inv, tmin, tcur, t1);
+    bool e2 = f2 != AURORA_BVH4_EMPTY && slab(l2, h2, o, inv, tmin, tcur, t2);
+    bool e3 = f3 != AURORA_BVH4_EMPTY && slab(l3, h3, o, inv, tmin, tcur, t3);
+    if (e0 && (f0 & AURORA_NODE_LEAF)) { test_prim(f0 & ~AURORA_NODE_LEAF); e0 = false; }
+    if (e1 && (f1 & AURORA_NODE_LEAF)) { test_prim(f1 & ~AURORA_NODE_LEAF); e1 = false; }
+    if (e2 && (f2 & AURORA_NODE_LEAF)) { test_prim(f2 & ~AURORA_NODE_LEAF); e2 = false; }
+    if (e3 && (f3 & AURORA_NODE_LEAF)) { test_prim(f3 & ~AURORA_NODE_LEAF); e3 = false; }
+    uint64_t k0 = e0 ? wide_key(t0, 0) : ~0ull, k1 = e1 ? wide_key(t1, 1) : ~0ull;
+    uint64_t k2 = e2 ? wide_key(t2, 2) : ~0ull, k3 = e3 ? wide_key(t3, 3) : ~0ull;
+    wide_cs(k0, k1);
+    wide_cs(k2, k3);
+    wide_cs(k0, k2);
+    wide_cs(k1, k3);
+    wide_cs(k1, k2);
+    auto ref_of = [&](uint64_t k) {
+        const uint32_t sl = (uint32_t)k & 3u;
+        return sl == 0 ? f0 : sl == 1 ? f1 : sl == 2 ? f2 : f3;
+    };
+    r0 = ref_of(k0);
+    r1 = ref_of(k1);
+    r2 = ref_of(k2);
+    r3 = ref_of(k3);
+    return (int)e0 + (int)e1 + (int)e2 + (int)e3;
+}
+
+// Binary form: one step fetches the aligned child pair at `cur` (two node
+// records, 64 bytes), tests both slabs, tests leaf primitives at once and
+// descends to the nearer child, pushing the farther.  Used where traversal
+// is issue bound (render bounces), where its cheaper step wins.
 // stk: this thread's LDS column (level k at stk[k * col_stride]); spill: its
 // global column for levels >= kTravLds (level k at spill[(k-kTravLds)*sstride]).
-__device__ __forceinline__ void traverse_closest(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
+__device__ __forceinline__ void traverse_closest_binary(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
                                                  uint32_t* stk, int col_stride, uint32_t* spill, size_t sstride,
                                                  float& t_out, int& prim_out) {
     // scene fields in registers: the spill stores below could otherwise

@@ -125,11 +125,27 @@ typedef struct aurora_bvh_node {
     uint32_t parent;  /* index of the parent node (root: 0xffffffff)     */
 } aurora_bvh_node;
 
+/* Four-wide node of the traversal tree (128 bytes, one cache line): the
+ * binary LBVH collapsed so that each node holds the grandchildren of an
+ * even-depth binary node (a child that is a leaf stays as itself).  ref is
+ * AURORA_NODE_LEAF | primitive, the index of another four-wide node, or
+ * AURORA_BVH4_EMPTY.  Node 0 is the root. */
+#define AURORA_BVH4_EMPTY 0xffffffffu
+typedef struct aurora_bvh4_slot {
+    float lo[3];
+    uint32_t ref;
+    float hi[3];
+    uint32_t pad;           /* 0                                            */
+} aurora_bvh4_slot;
+typedef struct aurora_bvh4_node {
+    aurora_bvh4_slot slot[4];
+} aurora_bvh4_node;
+
 typedef struct aurora_bvh_stats {
     uint32_t n_prims;
     uint32_t n_nodes;       /* 2*n_prims (1 for a single primitive)         */
     uint32_t depth;         /* longest root-to-leaf path, in edges          */
-    uint32_t reserved;
+    uint32_t n_wide;        /* four-wide traversal nodes                    */
     float scene_lo[3], scene_hi[3];     /* union of primitive boxes         */
     float centroid_lo[3], centroid_hi[3];/* Morton quantisation frame        */
     double build_ms;        /* device time of the build kernels            */
@@ -225,12 +241,16 @@ int aurora_scene_upload(aurora_ctx* ctx, const aurora_scene_desc* desc,
 void aurora_scene_destroy(aurora_scene* scene);
 
 /* Build the LBVH on the device: Morton codes, LDS radix sort, Karras radix
- * tree, bottom-up refit into 32-byte nodes.  No reference counterpart (ARE
- * scans linearly, experiments/rt.cpp:209-218). */
+ * tree, bottom-up refit into 32-byte nodes, then the four-wide traversal
+ * tree collapsed from it.  No reference counterpart (ARE scans linearly,
+ * experiments/rt.cpp:209-218). */
 int aurora_bvh_build(aurora_scene* scene, aurora_bvh_stats* stats);
 /* Copy the node array (stats.n_nodes nodes) to host memory. */
 int aurora_bvh_download(aurora_scene* scene, aurora_bvh_node* nodes,
                         uint32_t capacity);
+/* Copy the four-wide traversal nodes (stats.n_wide of them) to host memory. */
+int aurora_bvh_download_wide(aurora_scene* scene, aurora_bvh4_node* nodes,
+                             uint32_t capacity);
 /* Copy the sorted 30-bit Morton codes and primitive order to host memory. */
 int aurora_bvh_download_order(aurora_scene* scene, uint32_t* morton,
                               uint32_t* prim_order, uint32_t capacity);

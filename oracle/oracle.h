@@ -34,8 +34,11 @@ void orc_scene_destroy(orc_scene* s);
 int orc_bvh_build(orc_scene* s, aurora_bvh_stats* st);
 int orc_bvh_nodes(orc_scene* s, aurora_bvh_node* nodes, uint32_t cap);
 int orc_bvh_order(orc_scene* s, uint32_t* morton, uint32_t* order, uint32_t cap);
+int orc_bvh_wide(orc_scene* s, aurora_bvh4_node* nodes, uint32_t cap);
 void orc_prim_box(orc_scene* s, uint32_t id, float* lo, float* hi);
-/* counts: 3 per ray (nodes, spheres, quads fetched), BVH mode only */
+/* brute: 0 = the four-wide tree, 1 = every primitive, 2 = the binary tree.
+ * counts: 3 per ray (node records, spheres, quads fetched; a four-wide node
+ * is 4 node records), tree modes only. */
 int orc_trace(orc_scene* s, const float* o, const float* d, uint32_t n, float tmin, float tmax, int brute,
               int32_t* prim, float* t, uint32_t* counts, int threads);
 int orc_hit_sphere(const float* sphere, const float* o, const float* d, float tmin, float tmax, float* t);

@@ -40,6 +40,7 @@ def lib():
             "orc_bvh_build": (C.c_int, [vp, C.POINTER(abi.BvhStats)]),
             "orc_bvh_nodes": (C.c_int, [vp, vp, u32]),
             "orc_bvh_order": (C.c_int, [vp, vp, vp, u32]),
+            "orc_bvh_wide": (C.c_int, [vp, vp, u32]),
             "orc_prim_box": (None, [vp, u32, vp, vp]),
             "orc_trace": (C.c_int, [vp, vp, vp, u32, f, f, C.c_int, vp, vp, vp, C.c_int]),
             "orc_hit_sphere": (C.c_int, [vp, vp, vp, f, f, vp]),
@@ -175,6 +176,13 @@ class OracleScene:
         lib().orc_bvh_nodes(self.h, buf.ctypes.data, n)
         return buf[:n]
 
+    def wide(self):
+        """Four-wide traversal nodes as (n_wide, 32) uint32 words."""
+        n = self.stats["n_wide"]
+        buf = np.zeros((max(n, 1), 32), np.uint32)
+        lib().orc_bvh_wide(self.h, buf.ctypes.data, n)
+        return buf[:n]
+
     def order(self):
         n = self.arrays.n_prims
         m = np.zeros(max(n, 1), np.uint32)
@@ -188,14 +196,16 @@ class OracleScene:
         lib().orc_prim_box(self.h, i, lo.ctypes.data, hi.ctypes.data)
         return lo, hi
 
-    def trace(self, origins, dirs, tmin=0.0, tmax=np.inf, brute=False, counts=False, threads=0):
+    def trace(self, origins, dirs, tmin=0.0, tmax=np.inf, brute=False, counts=False, threads=0, binary=False):
+        """Closest hits over the four-wide tree (default), the binary tree
+        (binary=True) or every primitive (brute=True)."""
         o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
         d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
         n = len(o)
         prim = np.zeros(max(n, 1), np.int32)
         t = np.zeros(max(n, 1), np.float32)
         cnt = np.zeros((max(n, 1), 3), np.uint32) if counts else None
-        rc = lib().orc_trace(self.h, o.ctypes.data, d.ctypes.data, n, tmin, tmax, 1 if brute else 0,
+        rc = lib().orc_trace(self.h, o.ctypes.data, d.ctypes.data, n, tmin, tmax, 1 if brute else 2 if binary else 0,
                              prim.ctypes.data, t.ctypes.data, cnt.ctypes.data if counts else None, threads)
         if rc:
             raise RuntimeError(_err())

@@ -360,6 +360,12 @@ int orc_bvh_nodes(orc_scene* sc, aurora_bvh_node* nodes, uint32_t cap) {
     return AURORA_OK;
 }
 
+int orc_bvh_wide(orc_scene* sc, aurora_bvh4_node* nodes, uint32_t cap) {
+    if (cap < sc->s.wide.size()) { g_err = "capacity too small"; return AURORA_ERR_INVALID; }
+    std::memcpy(nodes, sc->s.wide.data(), sc->s.wide.size() * sizeof(aurora_bvh4_node));
+    return AURORA_OK;
+}
+
 int orc_bvh_order(orc_scene* sc, uint32_t* morton, uint32_t* order, uint32_t cap) {
     if (cap < sc->s.order.size()) { g_err = "capacity too small"; return AURORA_ERR_INVALID; }
     std::memcpy(morton, sc->s.sorted_morton.data(), sc->s.order.size() * 4);
@@ -376,11 +382,12 @@ int orc_trace(orc_scene* sc, const float* o, const float* d, uint32_t n, float t
         V3<float> oo = mk(o[3 * i], o[3 * i + 1], o[3 * i + 2]);
         V3<float> dd = mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
         Hit<float> h;
-        if (brute) {
+        if (brute == 1) {
             h = closest_brute<float>(sc->s, oo, dd, tmin, tmax);
         } else {
             Counts c;
-            h = closest_bvh<float>(sc->s, oo, dd, tmin, tmax, &c);
+            h = brute == 2 ? closest_bvh<float>(sc->s, oo, dd, tmin, tmax, &c)
+                           : closest_bvh4<float>(sc->s, oo, dd, tmin, tmax, &c);
             if (counts) { counts[3 * i] = c.nodes; counts[3 * i + 1] = c.spheres; counts[3 * i + 2] = c.quads; }
         }
         prim[i] = h.prim;

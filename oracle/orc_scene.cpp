@@ -201,9 +201,68 @@ static uint32_t quantise(float c, float lo, float hi) {
     return uint32_t(q);
 }
 
+// Four-wide traversal tree.  Every internal binary node of even depth
+// becomes one wide node, numbered in Karras index order; its slots are the
+// expansions of its two children in order (a leaf child is itself, an
+// internal child contributes its two children).  A slot that is an internal
+// binary node refers to that node's wide index (its depth is even again).
+static void collapse_wide(Scene& s, const uint32_t* gamma, const uint8_t* lleaf, const uint8_t* rleaf,
+                          const uint32_t* idepth, uint32_t ni) {
+    s.wide.clear();
+    const uint32_t n = s.n_prims();
+    auto slot_of = [&](uint32_t pos, uint32_t ref) {
+        aurora_bvh4_slot sl{};
+        for (int k = 0; k < 3; ++k) { sl.lo[k] = s.nodes[pos].lo[k]; sl.hi[k] = s.nodes[pos].hi[k]; }
+        sl.ref = ref;
+        sl.pad = 0;
+        return sl;
+    };
+    aurora_bvh4_slot empty{};
+    empty.ref = AURORA_BVH4_EMPTY;
+    if (n == 0) { s.stats.n_wide = 0; return; }
+    if (n == 1) {
+        aurora_bvh4_node w{};
+        w.slot[0] = slot_of(0, s.nodes[0].child);
+        for (int k = 1; k < 4; ++k) w.slot[k] = empty;
+        s.wide.push_back(w);
+        s.stats.n_wide = 1;
+        return;
+    }
+    std::vector<uint32_t> widx(ni, 0xffffffffu);
+    uint32_t nw = 0;
+    for (uint32_t i = 0; i < ni; ++i)
+        if ((idepth[i] & 1u) == 0) widx[i] = nw++;
+    s.wide.assign(nw, aurora_bvh4_node{});
+    for (uint32_t i = 0; i < ni; ++i) {
+        if (widx[i] == 0xffffffffu) continue;
+        aurora_bvh4_node& w = s.wide[widx[i]];
+        int k = 0;
+        const uint32_t g = gamma[i];
+        // children of internal i: positions 2+2g (internal index g) and 3+2g (g+1)
+        for (int c = 0; c < 2; ++c) {
+            const uint32_t pos = 2 + 2 * g + c;
+            const bool leaf = c == 0 ? lleaf[i] : rleaf[i];
+            if (leaf) {
+                w.slot[k++] = slot_of(pos, s.nodes[pos].child);
+                continue;
+            }
+            const uint32_t j = g + c;  // internal index of the child
+            const uint32_t gj = gamma[j];
+            for (int e = 0; e < 2; ++e) {
+                const uint32_t gpos = 2 + 2 * gj + e;
+                const bool gleaf = e == 0 ? lleaf[j] : rleaf[j];
+                w.slot[k++] = slot_of(gpos, gleaf ? s.nodes[gpos].child : widx[gj + e]);
+            }
+        }
+        for (; k < 4; ++k) w.slot[k] = empty;
+    }
+    s.stats.n_wide = nw;
+}
+
 void build_lbvh(Scene& s) {
     const uint32_t n = s.n_prims();
     s.nodes.clear();
+    s.wide.clear();
     s.sorted_morton.clear();
     s.order.clear();
     s.stats = aurora_bvh_stats{};
@@ -259,6 +318,7 @@ void build_lbvh(Scene& s) {
     if (n == 1) {
         s.nodes[0] = leaf_node(0, 0xffffffffu);
         s.stats.depth = 0;
+        collapse_wide(s, nullptr, nullptr, nullptr, nullptr, 0);
         return;
     }
 
@@ -328,6 +388,7 @@ void build_lbvh(Scene& s) {
         nd.parent = i == 0 ? 0xffffffffu : int_pos[int_parent[i]];
     }
     s.stats.depth = maxdepth;
+    collapse_wide(s, gamma.data(), lleaf.data(), rleaf.data(), idepth.data(), ni);
 }
 
 // ----------------------------------------------------- primitive tests ---
@@ -458,6 +519,62 @@ Hit<R> closest_bvh(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt
 }
 
 template <class R>
+static inline bool slab4(const aurora_bvh4_slot& sl, V3<R> o, V3<R> inv, R tmin, R tcur, R& tnear) {
+    aurora_bvh_node nd{};
+    for (int k = 0; k < 3; ++k) { nd.lo[k] = sl.lo[k]; nd.hi[k] = sl.hi[k]; }
+    return slab(nd, o, inv, tmin, tcur, tnear);
+}
+
+// One step tests the four slots against the current best t, tests the hit
+// leaf slots' primitives in slot order, then orders the hit internal slots
+// by entry distance (ties: slot order), descends to the nearest and pushes
+// the others farthest first.
+template <class R>
+Hit<R> closest_bvh4(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt) {
+    Hit<R> best{tmax, 0x7fffffff};
+    if (!s.built) throw std::logic_error("BVH not built");
+    if (s.wide.empty()) { best.prim = AURORA_MISS; return best; }
+    V3<R> inv = mk<R>(R(1) / d.x, R(1) / d.y, R(1) / d.z);
+    uint32_t stack[96];  // three pushes per level of a tree of depth <= 64
+    int sp = 0;
+    uint32_t cur = 0;
+    for (;;) {
+        if (cur >= s.wide.size()) throw std::logic_error("malformed four-wide tree");
+        const aurora_bvh4_node& w = s.wide[cur];
+        if (cnt) cnt->nodes += 4;
+        bool hit[4];
+        R tn[4];
+        for (int k = 0; k < 4; ++k) {
+            tn[k] = R(0);
+            hit[k] = w.slot[k].ref != AURORA_BVH4_EMPTY && slab4(w.slot[k], o, inv, tmin, best.t, tn[k]);
+        }
+        for (int k = 0; k < 4; ++k)
+            if (hit[k] && (w.slot[k].ref & AURORA_NODE_LEAF)) {
+                test_prim(s, w.slot[k].ref & ~AURORA_NODE_LEAF, o, d, tmin, tmax, best, cnt);
+                hit[k] = false;
+            }
+        // stable order of the hit internal slots by entry distance
+        int ord[4], m = 0;
+        for (int k = 0; k < 4; ++k)
+            if (hit[k]) ord[m++] = k;
+        for (int a = 1; a < m; ++a)
+            for (int b = a; b > 0 && tn[ord[b]] < tn[ord[b - 1]]; --b) std::swap(ord[b], ord[b - 1]);
+        if (m == 0) {
+            if (sp == 0) break;
+            cur = stack[--sp];
+            continue;
+        }
+        for (int k = m - 1; k >= 1; --k) {
+            if (sp >= 96) throw std::logic_error("four-wide traversal stack overflow");
+            stack[sp++] = w.slot[ord[k]].ref;
+        }
+        cur = w.slot[ord[0]].ref;
+    }
+    if (best.prim == 0x7fffffff) { best.prim = AURORA_MISS; best.t = tmax; }
+    return best;
+}
+
+template <class R>
 Hit<R> closest_brute(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax) {
     Hit<R> best{tmax, 0x7fffffff};
     for (uint32_t id = 0; id < s.n_prims(); ++id) test_prim<R>(s, id, o, d, tmin, tmax, best, nullptr);
@@ -554,6 +671,7 @@ V3<R> texture_value(const Scene& s, uint32_t ti, R u, R v, V3<R> p) {
     template bool hit_sphere<R>(const Sphere&, V3<R>, V3<R>, R, R, R&);                               \
     template bool hit_quad<R>(const Quad&, V3<R>, V3<R>, R, R, R&, R&, R&);                            \
     template Hit<R> closest_bvh<R>(const Scene&, V3<R>, V3<R>, R, R, Counts*);                         \
+    template Hit<R> closest_bvh4<R>(const Scene&, V3<R>, V3<R>, R, R, Counts*);                        \
     template Hit<R> closest_brute<R>(const Scene&, V3<R>, V3<R>, R, R);                                \
     template V3<R> texture_value<R>(const Scene&, uint32_t, R, R, V3<R>);                              \
     template R perlin_noise<R>(const Perlin&, V3<R>);                                                  \

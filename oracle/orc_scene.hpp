@@ -32,6 +32,7 @@ struct Scene {
     std::vector<Perlin> perlin;        // one per texture (only noise ones used)
     // LBVH
     std::vector<aurora_bvh_node> nodes;
+    std::vector<aurora_bvh4_node> wide;  // four-wide traversal tree
     std::vector<uint32_t> sorted_morton, order;
     aurora_bvh_stats stats{};
     bool built = false;
@@ -58,6 +59,9 @@ struct Hit {
 // closest hit of (o, d) in (tmin, tmax); ties go to the lower id.
 template <class R>
 Hit<R> closest_bvh(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt);
+// closest hit over the four-wide tree (counts: 4 node records per wide node)
+template <class R>
+Hit<R> closest_bvh4(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt);
 template <class R>
 Hit<R> closest_brute(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax);
 

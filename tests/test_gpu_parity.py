@@ -39,13 +39,14 @@ def test_lbvh_bit_exact(ctx, name):
     say("lbvh", name)
     arr = SCENES[name]()
     g, gs, o, os_ = both(ctx, arr)
-    for k in ("n_prims", "n_nodes", "depth", "scene_lo", "scene_hi", "centroid_lo", "centroid_hi"):
+    for k in ("n_prims", "n_nodes", "n_wide", "depth", "scene_lo", "scene_hi", "centroid_lo", "centroid_hi"):
         assert gs[k] == os_[k], k
     gm, go = g.order()
     om, oo = o.order()
     assert np.array_equal(gm, om)
     assert np.array_equal(go, oo)
     assert np.array_equal(g.nodes(), o.nodes())
+    assert np.array_equal(g.wide(), o.wide())
 
 
 @pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 100])
@@ -60,9 +61,10 @@ def test_lbvh_small_and_duplicates(ctx, n):
         b.sphere(c, 0.2, m)
     arr = b.arrays() if n else SceneArrays()
     g, gs, o, os_ = both(ctx, arr)
-    assert gs["n_nodes"] == os_["n_nodes"]
+    assert gs["n_nodes"] == os_["n_nodes"] and gs["n_wide"] == os_["n_wide"]
     if arr.n_prims:
         assert np.array_equal(g.nodes(), o.nodes())
+        assert np.array_equal(g.wide(), o.wide())
         assert np.array_equal(g.order()[1], o.order()[1])
 
 
@@ -235,6 +237,8 @@ def test_stress_full_scale_traversal(ctx):
     say("stress full scale: lbvh compare")
     assert np.array_equal(g.order()[1], o.order()[1])
     assert np.array_equal(g.nodes(), o.nodes())
+    assert gs["n_wide"] == os_["n_wide"]
+    assert np.array_equal(g.wide(), o.wide())
     n = 1 << 24
     sub = np.arange(0, n, 256)
     say("stress full scale: rays")

@@ -72,6 +72,58 @@ def check_topology(nodes, n, lo, hi):
     return depth
 
 
+def even_depth_internal_count(nodes):
+    """Internal binary nodes at even depth, from the parent links."""
+    child = nodes[:, 3]
+    count, stack = 0, [(0, 0)]
+    while stack:
+        i, d = stack.pop()
+        c = int(child[i])
+        if c & LEAF:
+            continue
+        count += d % 2 == 0
+        stack += [(c, d + 1), (c + 1, d + 1)]
+    return count
+
+
+def check_wide(wide, nodes, n, st):
+    """Structural invariants of the four-wide tree: every primitive once as
+    a leaf slot, every wide node but the root referenced once, valid slots
+    first, a referenced node's slot boxes contained in the referring slot
+    and exactly covering it, the root covering the scene, and one wide node
+    per even-depth internal binary node."""
+    assert len(wide) == st["n_wide"]
+    if n == 0:
+        assert len(wide) == 0
+        return
+    w = wide.reshape(len(wide), 4, 8)
+    ref = w[:, :, 3]
+    f = w.view(np.float32)
+    slo, shi = f[:, :, 0:3], f[:, :, 4:7]
+    assert np.all(w[:, :, 7] == 0)
+    valid = ref != NONE
+    for k in range(3):
+        assert np.all(valid[:, k] | ~valid[:, k + 1])  # empty slots only at the end
+    leaves = [int(r) & ~LEAF for r in ref[valid] if int(r) & LEAF]
+    assert sorted(leaves) == list(range(n))
+    inner = [int(r) for r in ref[valid] if not int(r) & LEAF]
+    assert sorted(inner) == list(range(1, len(wide)))
+    for i in range(len(wide)):
+        v = valid[i]
+        assert v.sum() >= (1 if n == 1 else 2)
+        for k in np.nonzero(v)[0]:
+            r = int(ref[i, k])
+            if r & LEAF:
+                continue
+            cv = valid[r]
+            assert np.all(slo[r][cv] >= slo[i, k]) and np.all(shi[r][cv] <= shi[i, k])
+            assert np.array_equal(slo[r][cv].min(0), slo[i, k]) and np.array_equal(shi[r][cv].max(0), shi[i, k])
+    assert np.array_equal(slo[0][valid[0]].min(0), np.array(st["scene_lo"], np.float32))
+    assert np.array_equal(shi[0][valid[0]].max(0), np.array(st["scene_hi"], np.float32))
+    if n > 1:
+        assert len(wide) == even_depth_internal_count(nodes)
+
+
 def build(arr):
     sc = orc.OracleScene(arr)
     st = sc.build_bvh()
@@ -94,6 +146,7 @@ def test_morton_order_topology(name, arr):
     assert np.array_equal(m, code[order])
     depth = check_topology(sc.nodes(), n, lo, hi)
     assert depth == st["depth"]
+    check_wide(sc.wide(), sc.nodes(), n, st)
     assert st["n_nodes"] == 2 * n and st["n_prims"] == n
     assert np.array_equal(np.array(st["scene_lo"], np.float32), lo.min(0))
     assert np.array_equal(np.array(st["scene_hi"], np.float32), hi.max(0))
@@ -114,6 +167,7 @@ def test_tiny_and_duplicate_scenes(n):
         sc, st = build(arr)
         code, lo, hi = morton_np(sc)
         check_topology(sc.nodes(), n, lo, hi)
+        check_wide(sc.wide(), sc.nodes(), n, st)
         o, d = scenes.random_rays(500, [-2, -2, -2], [2, 2, 2], seed=n)
         pb, tb = sc.trace(o, d)
         pf, tf = sc.trace(o, d, brute=True)
@@ -141,8 +195,12 @@ def test_bvh_equals_brute_force(name, arr):
     assert (pb >= 0).mean() > 0.2
     assert np.array_equal(pb, pf)
     assert np.array_equal(tb, tf)
-    # each ray fetches the root and whole child pairs
-    assert np.all(cnt[:, 0] % 2 == 1)
+    # each ray fetches whole four-wide nodes (4 node records each), the root first
+    assert np.all(cnt[:, 0] % 4 == 0) and np.all(cnt[:, 0] >= 4)
+    # the binary tree gives the same hits; it fetches the root, then child pairs
+    pn, tn, cb = sc.trace(o, d, tmin=1e-4, counts=True, binary=True)
+    assert np.array_equal(pn, pf) and np.array_equal(tn, tf)
+    assert np.all(cb[:, 0] % 2 == 1)
     ns = len(arr.spheres)
     assert np.all(cnt[pb >= 0].sum(1) > 1)
     if ns == 0:
