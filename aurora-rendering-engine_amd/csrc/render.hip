@@ -84,14 +84,36 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Pass pa, float4* qo, float4* 
     L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// sin and cos of 2*pi*u for u in [0, 1) (u a multiple of 2^-24): exact
+// reduction to an eighth of a turn (8u, its integer and fractional parts and
+// 1 - f are exact), then Taylor polynomials on [0, pi/4] (truncation below
+// 2e-9), folded back by octant symmetry
+__device__ __forceinline__ void sincos_turn(float u, float& s, float& c) {
+    const float t = 8.0f * u;
+    const float fk = floorf(t);
+    const int k = (int)fk;
+    const float f = t - fk;
+    const bool odd = (k & 1) != 0;
+    const float a = (odd ? 1.0f - f : f) * 0.785398163f;
+    const float a2 = a * a;
+    const float sp = a + (a * a2) * (-0.166666672f + a2 * (0.00833333377f + a2 * (-0.000198412701f + a2 * 2.75573188e-6f)));
+    const float cp =
+        1.0f + a2 * (-0.5f + a2 * (0.0416666679f + a2 * (-0.00138888892f + a2 * (2.48015876e-5f + a2 * -2.75573188e-7f))));
+    const float sa = odd ? cp : sp, ca = odd ? sp : cp;  // sin, cos of the angle within the quadrant
+    const int q = k >> 1;
+    s = q == 0 ? sa : q == 1 ? ca : q == 2 ? -sa : -ca;
+    c = q == 0 ? ca : q == 1 ? -sa : q == 2 ? -ca : sa;
+}
+
+// uniform direction on the unit sphere from one Philox draw, without
+// rejection: z uniform in (-1, 1] (exact), azimuth uniform (Archimedes)
 __device__ __forceinline__ v3 random_unit(uint32_t pixel, uint32_t sample, uint32_t bounce, uint32_t seed) {
-    for (uint32_t k = 0; k < 64; ++k) {
-        const u4 c = philox(u4{pixel, sample, bounce, k}, seed, STREAM_RENDER);
-        const v3 p = V(2.0f * u01(c.x) - 1.0f, 2.0f * u01(c.y) - 1.0f, 2.0f * u01(c.z) - 1.0f);
-        const float l2 = dot(p, p);
-        if (l2 > 1e-30f && l2 <= 1.0f) return p / sqrtf(l2);
-    }
-    return V(0.0f, 0.0f, 1.0f);
+    const u4 r = philox(u4{pixel, sample, bounce, 0u}, seed, STREAM_RENDER);
+    const float z = 1.0f - 2.0f * u01(r.x);
+    const float rho = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float sn, cs;
+    sincos_turn(u01(r.y), sn, cs);
+    return V(rho * cs, rho * sn, z);
 }
 
 __device__ __forceinline__ float schlick(float cosine, float ri) {

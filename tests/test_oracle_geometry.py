@@ -118,6 +118,22 @@ def test_random_unit_is_unit_and_isotropic():
     assert np.allclose(m2, np.eye(3) / 3, atol=0.04)
 
 
+def test_random_unit_is_the_archimedes_map_of_its_draw():
+    # one Philox draw (pixel, sample, bounce, 0) with key (seed, render
+    # stream): z = 1 - 2 u1, azimuth 2 pi u2, compared with libm
+    from aurora_amd.scenes import philox4x32_10
+    pix = np.arange(3000)
+    x, y, _, _ = philox4x32_10(pix, 5, 2, 0, 9, 0)
+    u1 = (x >> 8).astype(np.float64) / 16777216.0
+    u2 = (y >> 8).astype(np.float64) / 16777216.0
+    z = 1.0 - 2.0 * u1
+    rho = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    want = np.stack([rho * np.cos(2 * np.pi * u2), rho * np.sin(2 * np.pi * u2), z], 1)
+    got = np.array([orc.random_unit(int(p), 5, 2, seed=9) for p in pix], np.float64)
+    assert np.array_equal(got[:, 2], z)  # exact
+    assert np.max(np.abs(got - want)) < 3e-7
+
+
 def test_lambertian_cosine_distribution_chi2():
     # n + random_unit, normalised, is cosine-distributed: cos(theta) = sqrt(U)
     # => cos^2 uniform on [0,1].  Chi-square over 10 bins, 5000 samples.
