@@ -203,12 +203,21 @@ __global__ void k_db_partial(const u16* __restrict__ g, int cout, size_t npix, s
         partial[(size_t)blockIdx.x * cout + threadIdx.x] = t;
     }
 }
+// one block per channel: strided partial sums, then a fixed-shape tree (the
+// order of additions does not depend on timing, so the result is
+// deterministic)
 __global__ void k_db_reduce(const float* __restrict__ partial, int nblk, int cout, float* __restrict__ db) {
-    const int co = threadIdx.x;
-    if (co >= cout) return;
+    __shared__ float red[256];
+    const int co = blockIdx.x;
     float s = 0.0f;
-    for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * cout + co];
-    db[co] = s;
+    for (int b = threadIdx.x; b < nblk; b += 256) s += partial[(size_t)b * cout + co];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[co] = red[0];
 }
 
 // ------------------------------------------------------- elementwise ---
@@ -455,7 +464,8 @@ Status conv3x3_weight_grad(const ConvIn& in, const uint16_t* g, int cout, int N,
     const int rows_per_block = 16;
     const int nblk = (rows + rows_per_block - 1) / rows_per_block;
     const size_t npix = (size_t)N * H * W;
-    const size_t per_block = 4096;
+    // bias gradient: 512 pixels per block (a 128x128x8 batch is 256 blocks)
+    const size_t per_block = 512;
     const int nbb = (int)((npix + per_block - 1) / per_block);
     AUR_TRY(scratch.alloc(((size_t)nblk * ntiles * 256 + (size_t)nbb * cout) * 4));
     float* part = scratch.as<float>();
@@ -463,7 +473,7 @@ Status conv3x3_weight_grad(const ConvIn& in, const uint16_t* g, int cout, int N,
     k_conv3x3_dw<<<dim3(nblk, ntiles), 256, 0, s>>>(in, g, cout, N, H, W, rows_per_block, part);
     k_dw_reduce<<<nblocks((size_t)ntiles * 256), 256, 0, s>>>(part, nblk, ntiles, cout, cin, dw);
     k_db_partial<<<nbb, 256, 0, s>>>(g, cout, npix, per_block, dbpart);
-    k_db_reduce<<<1, 64, 0, s>>>(dbpart, nbb, cout, db);
+    k_db_reduce<<<cout, 256, 0, s>>>(dbpart, nbb, cout, db);
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }

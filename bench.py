@@ -116,7 +116,7 @@ TAIL_ALWAYS = (1 << 32) - 1  # fused-tail switch above any queue: one persistent
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain 1 GiB copy
 # gives), keyed (kernel, width, tail switch); the summaries and commands
 # are in the profiles/ directory.
-PMC_TRAFFIC = {("k_tail", 1024, TAIL_ALWAYS): 667.9e6}  # profiles/tail_cornell_1024x1024.md
+PMC_TRAFFIC = {("k_tail", 1024, TAIL_ALWAYS): 668.4e6}  # profiles/tail_cornell_1024x1024.md
 
 
 def note(msg):
@@ -195,6 +195,40 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=4096):
     return out
 
 
+def denoiser_section(ctx, size=1024, batch=8, crop=128, reps=10):
+    """Denoiser throughput: one inference on a size x size image (3x3 conv
+    FLOPs of the 15 layers, 2*9*Cin*Cout per pixel at each level's
+    resolution, over the measured time) and one training step (forward,
+    backward, Adam) on batch x crop x crop."""
+    import torch
+
+    from aurora_amd.denoiser import LAYERS, SLOT_L, Denoiser, layer_shapes
+
+    flop_px = sum(2 * 9 * ci * co / 4 ** SLOT_L[o] for (_, _, o, _), (co, ci) in zip(LAYERS, layer_shapes()))
+    dn = Denoiser(ctx, seed=1)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    img = [torch.rand((size, size, 3), generator=g, device="cuda") * 16 for _ in range(3)]
+    dn.denoise(*img, 16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dn.denoise(*img, 16)
+    inf_s = (time.perf_counter() - t0) / reps
+    tr = [torch.rand((batch, crop, crop, 3), generator=g, device="cuda") * 16 for _ in range(4)]
+    dn.train_step(*tr[:3], tr[3], 16, 1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dn.train_step(*tr[:3], tr[3], 16, 1e-4)
+    tr_s = (time.perf_counter() - t0) / reps
+    dn.close()
+    inf_flop = flop_px * size * size
+    tr_flop = 3 * flop_px * batch * crop * crop  # forward, data gradient, weight gradient
+    return {"flop_per_pixel": flop_px, "inference_px": size * size, "inference_ms": inf_s * 1e3,
+            "inference_TFLOPs": inf_flop / inf_s / 1e12, "train_step_px": batch * crop * crop,
+            "train_step_ms": tr_s * 1e3, "train_TFLOPs": tr_flop / tr_s / 1e12, "peak_bf16_TFLOPs": 2500.0}
+
+
 def serving_section(ctx, n_scenes=64, seed0=0):
     """Batched serving: n_scenes seeded scenes of uneven size (10..200k
     primitives, 256^2..1024^2, 1..16 spp) rendered in shared passes with an
@@ -268,7 +302,8 @@ def main():
     ap.add_argument("--no-stress", action="store_true", help="skip the traversal stress scene")
     ap.add_argument("--no-metric", action="store_true", help="skip time-to-relMSE (plain vs denoised)")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving section")
-    ap.add_argument("--only", choices=["stress", "copy", "serving"], default=None,
+    ap.add_argument("--no-denoiser", action="store_true", help="skip the denoiser throughput section")
+    ap.add_argument("--only", choices=["stress", "copy", "serving", "denoiser"], default=None,
                     help="profiling modes: the stress section alone, or a plain 1 GiB device copy "
                          "(calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -297,6 +332,9 @@ def main():
         return
     if args.only == "serving":
         print(json.dumps(serving_section(ctx)), flush=True)
+        return
+    if args.only == "denoiser":
+        print(json.dumps(denoiser_section(ctx)), flush=True)
         return
     if args.tail_paths is not None:
         ctx.set_launch(tail_paths=args.tail_paths)
@@ -419,6 +457,8 @@ def main():
     if not args.no_stress:
         scene.close()
         out["stress"] = stress_section(ctx)
+    if not args.no_denoiser:
+        out["denoiser"] = denoiser_section(ctx)
     if not args.no_serving:
         ctx.set_launch()
         out["serving"] = serving_section(ctx)
