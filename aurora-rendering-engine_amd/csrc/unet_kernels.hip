@@ -73,39 +73,64 @@ __device__ __forceinline__ u16 gather1(const ConvIn& in, int n, int y, int x, in
 }
 
 // ------------------------------------------------------------ forward ---
-// Block: 4 waves, 64 consecutive pixels of one image row; each wave a
-// 16-pixel x Cout tile (Cout/16 accumulators).  K = 9*Cin in chunks of 32.
+// Block: 4 waves; each wave a 64-pixel segment of one image row (4 MFMA row
+// tiles of 16 pixels) x Cout (NT column tiles of 16 channels), so every
+// weight fragment feeds 4 MFMAs and every input fragment NT.  K = 9*Cin in
+// chunks of 32; the next chunk's fragments are loaded before this chunk's
+// MFMAs issue, so the loads overlap the matrix work.
 template <int NT, bool RELU, bool OUT_F32, bool ACCUM>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvIn in, const u16* __restrict__ wk, int kpad,
                                                  const float* __restrict__ bias, void* out, int N, int H, int W,
                                                  int out_stride, int co_base) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row = blockIdx.x / ((W + 63) / 64);
-    const int xb = (blockIdx.x % ((W + 63) / 64)) * 64 + wave * 16;
+    const int segs = (W + 63) / 64;
+    const int seg = blockIdx.x * 4 + wave;
+    const int row = seg / segs;
+    const int x0 = (seg % segs) * 64;
     const int n = row / H, y = row % H;
-    if (n >= N || xb >= W) return;
+    if (n >= N) return;  // the whole wave: seg is wave-uniform
     const int cin = in.ca + in.cb;
     const int K = 9 * cin;
-    const int m = lane & 15;      // pixel of the A fragment
-    const int kg = lane >> 4;     // k group of 8
-    const int px = xb + m;
-    floatx4 acc[NT];
+    const int m = lane & 15;   // pixel of the A fragment within its row tile
+    const int kg = lane >> 4;  // k group of 8
+    floatx4 acc[4][NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < kpad; k0 += 32) {
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[mt][t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x8 a[4], b[NT];
+    auto load = [&](int k0, bf16x8* fa, bf16x8* fb) {
         const int k = k0 + 8 * kg;
-        bf16x8 a;
         if (k < K) {
             const int tap = k / cin, c = k - tap * cin;
-            a = gather8(in, n, y + tap / 3 - 1, px + tap % 3 - 1, H, W, c);
+            const int yy = y + tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) fa[mt] = gather8(in, n, yy, x0 + 16 * mt + m + dx, H, W, c);
         } else {
-            a = zero8();
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) fa[mt] = zero8();
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            Pack8 b;
-            b.u = *reinterpret_cast<const uint4*>(wk + (size_t)(t * 16 + m) * kpad + k);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b.v, acc[t], 0, 0, 0);
+            Pack8 w;
+            w.u = *reinterpret_cast<const uint4*>(wk + (size_t)(t * 16 + m) * kpad + k);
+            fb[t] = w.v;
+        }
+    };
+    load(0, a, b);
+    for (int k0 = 0; k0 < kpad; k0 += 32) {
+        bf16x8 na[4], nb[NT];
+        const bool more = k0 + 32 < kpad;
+        if (more) load(k0 + 32, na, nb);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[t], acc[mt][t], 0, 0, 0);
+        if (more) {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) a[mt] = na[mt];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) b[t] = nb[t];
         }
     }
 #pragma unroll
@@ -113,17 +138,20 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn in, const u16* __restric
         const int co = t * 16 + m;
         const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int x = xb + 4 * kg + r;
-            if (x >= W) continue;
-            float v = acc[t][r] + bv;
-            if (RELU) v = fmaxf(v, 0.0f);
-            const size_t o = (((size_t)n * H + y) * W + x) * out_stride + co_base + co;
-            if (OUT_F32) {
-                float* of = static_cast<float*>(out);
-                of[o] = ACCUM ? of[o] + v : v;
-            } else {
-                static_cast<u16*>(out)[o] = f2bf(v);
+        for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = x0 + 16 * mt + 4 * kg + r;
+                if (x >= W) continue;
+                float v = acc[mt][t][r] + bv;
+                if (RELU) v = fmaxf(v, 0.0f);
+                const size_t o = (((size_t)n * H + y) * W + x) * out_stride + co_base + co;
+                if (OUT_F32) {
+                    float* of = static_cast<float*>(out);
+                    of[o] = ACCUM ? of[o] + v : v;
+                } else {
+                    static_cast<u16*>(out)[o] = f2bf(v);
+                }
             }
         }
     }
@@ -425,7 +453,7 @@ Status conv3x3_forward(const ConvIn& in, const uint16_t* wk_all, int kpad, const
                        bool out_f32, bool accum, bool relu, int N, int H, int W, int cout, hipStream_t s) {
     if ((in.ca % 16) || (in.cb % 16) || (cout % 16) || cout < 16 || (in.ca + in.cb) < 16)
         return Status{AURORA_ERR_INVALID, "conv3x3: channel counts must be multiples of 16"};
-    const unsigned grid = (unsigned)(N * H * ((W + 63) / 64));
+    const unsigned grid = (unsigned)(((size_t)N * H * ((W + 63) / 64) + 3) / 4);  // 4 row segments per block
     // output channels in groups of at most 64 (4 MFMA tiles per wave)
     for (int co_base = 0; co_base < cout; co_base += 64) {
     const int nt = std::min(64, cout - co_base) / 16;
