@@ -116,6 +116,84 @@ def test_closest_hit_bit_exact(ctx, name, nrays):
         assert np.array_equal(dt.cpu().numpy(), ot)
 
 
+@pytest.mark.parametrize("name", ["cornell", "spheres"])
+def test_closest_hit_2_24_rays_against_brute_force(ctx, name):
+    """2^24 seeded rays on a small scene: the four-wide traversal (sorted,
+    as a query of this size runs) equals the GPU brute-force scan on every
+    ray, bit for bit; every 1024th ray is checked against the oracle."""
+    say("trace 2^24 vs brute", name)
+    arr = SCENES[name]()
+    g = ctx.upload(arr)
+    gs = g.build_bvh()
+    lo = np.array(gs["scene_lo"]) - 0.05 * (np.array(gs["scene_hi"]) - np.array(gs["scene_lo"]))
+    hi = np.array(gs["scene_hi"]) + 0.05 * (np.array(gs["scene_hi"]) - np.array(gs["scene_lo"]))
+    if name == "spheres":
+        lo, hi = [-12, 0.01, -12], [12, 3, 12]
+    n = 1 << 24
+    od, dd = scenes.random_rays_torch(n, list(lo), list(hi), seed=31)
+    bp = torch.empty(n, dtype=torch.int32, device="cuda")
+    bt = torch.empty(n, dtype=torch.float32, device="cuda")
+    fp = torch.empty_like(bp)
+    ft = torch.empty_like(bt)
+    ctx.set_ray_sort(2)
+    try:
+        g.trace_device(od, dd, bp, bt, tmin=1e-4)
+    finally:
+        ctx.set_ray_sort(1)
+    g.trace_device(od, dd, fp, ft, tmin=1e-4, brute=True)
+    ctx.synchronize()
+    assert (bp >= 0).float().mean().item() > 0.2
+    assert torch.equal(bp, fp)
+    assert torch.equal(bt.view(torch.int32), ft.view(torch.int32))
+    sub = torch.arange(0, n, 1024, device="cuda")
+    o = orc.OracleScene(arr)
+    o.build_bvh()
+    op, ot = o.trace(od[sub].cpu().numpy(), dd[sub].cpu().numpy(), tmin=1e-4)
+    assert np.array_equal(bp[sub].cpu().numpy(), op)
+    assert np.array_equal(bt[sub].cpu().numpy().view(np.uint32), ot.view(np.uint32))
+
+
+def test_closest_hit_axis_rays_and_rays_on_faces(ctx):
+    """Directions along the axes (1/d infinite, 0*inf in the slabs) and
+    origins on the planes of the Cornell walls and box faces: GPU queries,
+    GPU brute force and the oracle agree bit for bit."""
+    say("trace axis rays")
+    arr = scenes.cornell_box()
+    g, gs, o, os_ = both(ctx, arr)
+    rng = np.random.default_rng(12)
+    dirs, orgs = [], []
+    axes = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float32)
+    for k in range(6000):
+        d = axes[k % 6]
+        p = rng.uniform(-10, 565, 3).astype(np.float32)
+        # snap one coordinate onto a wall or box-face plane
+        p[rng.integers(3)] = np.float32(rng.choice([0.0, 555.0, 165.0, 330.0, 265.0, 295.0, 130.0, 65.0]))
+        orgs.append(p)
+        dirs.append(d)
+    for k in range(2000):  # axis-parallel in one coordinate only
+        d = rng.normal(size=3).astype(np.float32)
+        d[k % 3] = 0.0
+        d /= np.linalg.norm(d)
+        orgs.append(rng.uniform(-10, 565, 3).astype(np.float32))
+        dirs.append(d.astype(np.float32))
+    org = np.array(orgs, np.float32)
+    dirs = np.array(dirs, np.float32)
+    op, ot = o.trace(org, dirs, tmin=1e-4)
+    n = len(org)
+    dp = torch.zeros(n, dtype=torch.int32, device="cuda")
+    dt = torch.zeros(n, dtype=torch.float32, device="cuda")
+    g.trace_device(dev(org), dev(dirs), dp, dt, tmin=1e-4)
+    bp = torch.zeros_like(dp)
+    bt = torch.zeros_like(dt)
+    g.trace_device(dev(org), dev(dirs), bp, bt, tmin=1e-4, brute=True)
+    ctx.synchronize()
+    assert (op >= 0).mean() > 0.3
+    assert np.array_equal(dp.cpu().numpy(), op)
+    assert np.array_equal(dt.cpu().numpy().view(np.uint32), ot.view(np.uint32))
+    assert np.array_equal(bp.cpu().numpy(), op)
+    assert np.array_equal(bt.cpu().numpy().view(np.uint32), ot.view(np.uint32))
+
+
 RENDERS = [
     ("cornell", scenes.cornell_box, lambda: scenes.cornell_camera(48), RenderSettings(spp=12, max_depth=50)),
     ("spheres", lambda: scenes.random_spheres(0), lambda: scenes.random_spheres_camera(64, 36),
