@@ -159,6 +159,7 @@ EXPORTS = [
     ("aurora_stream", _vp, [_vp]),
     ("aurora_set_launch", C.c_int, [_vp, C.c_int, C.c_int, _u32]),
     ("aurora_set_ray_sort", C.c_int, [_vp, C.c_int]),
+    ("aurora_set_render_tree", C.c_int, [_vp, C.c_int]),
     ("aurora_scene_upload", C.c_int, [_vp, C.POINTER(SceneDesc), C.POINTER(_vp)]),
     ("aurora_scene_destroy", None, [_vp]),
     ("aurora_bvh_build", C.c_int, [_vp, C.POINTER(BvhStats)]),

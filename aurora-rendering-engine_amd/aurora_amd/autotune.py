@@ -4,9 +4,10 @@ A workload (one scene or a batch) is rendered once per candidate launch
 shape, and the fastest is kept and cached under a signature of the
 workload (pixels, samples, primitives).  The shape covers the persistent
 traversal and shading grids (blocks per compute unit), the live-path count
-at which the fused tail takes over, and whether passes are captured in a
-hipGraph.  Shapes change speed only, never results (tests check the images
-bit for bit), so the tuner can be run at any time.
+at which the fused tail takes over, whether passes are captured in a
+hipGraph, and which tree (binary or four-wide) the render kernels walk.
+Shapes change speed only, never results (tests check the images bit for
+bit), so the tuner can be run at any time.
 """
 import itertools
 import json
@@ -20,7 +21,16 @@ CANDIDATES = {
     "shade_blocks_per_cu": [8],
     "tail_paths": [1 << 20, (1 << 32) - 1],
     "graph": [False, True],
+    "wide": [False, True],
 }
+
+
+def apply(ctx, shape):
+    """Put a launch shape on the context (keys absent from it keep their
+    defaults)."""
+    ctx.set_launch(shape["extend_blocks_per_cu"], shape["shade_blocks_per_cu"], shape["tail_paths"])
+    if hasattr(ctx, "set_render_tree"):
+        ctx.set_render_tree(bool(shape.get("wide", False)))
 
 
 def signature(items):
@@ -43,14 +53,14 @@ def tune(ctx, run, sig, cache_path=DEFAULT_CACHE, candidates=None, repeats=2, lo
             cache = {}
     if sig in cache:
         best = cache[sig]["best"]
-        ctx.set_launch(best["extend_blocks_per_cu"], best["shade_blocks_per_cu"], best["tail_paths"])
+        apply(ctx, best)
         return best, cache[sig]["table"]
     cand = candidates or CANDIDATES
     keys = list(cand)
     table = []
     for values in itertools.product(*(cand[k] for k in keys)):
         shape = dict(zip(keys, values))
-        ctx.set_launch(shape["extend_blocks_per_cu"], shape["shade_blocks_per_cu"], shape["tail_paths"])
+        apply(ctx, shape)
         run(shape["graph"])  # warm-up (allocations, graph capture paths)
         ts = []
         for _ in range(repeats):
@@ -62,7 +72,7 @@ def tune(ctx, run, sig, cache_path=DEFAULT_CACHE, candidates=None, repeats=2, lo
         if log:
             log(f"autotune {sig}: {shape}")
     best = min(table, key=lambda s: s["seconds"])
-    ctx.set_launch(best["extend_blocks_per_cu"], best["shade_blocks_per_cu"], best["tail_paths"])
+    apply(ctx, best)
     if cache_path:
         cache[sig] = {"best": best, "table": table}
         os.makedirs(os.path.dirname(cache_path), exist_ok=True)

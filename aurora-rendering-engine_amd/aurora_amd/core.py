@@ -180,6 +180,11 @@ class Context:
     def set_launch(self, extend_blocks_per_cu=8, shade_blocks_per_cu=8, tail_paths=TAIL_ALWAYS):
         self.check(self.lib.aurora_set_launch(self.h, extend_blocks_per_cu, shade_blocks_per_cu, tail_paths))
 
+    def set_render_tree(self, wide=False):
+        """Render kernels walk the four-wide tree (True) or the binary LBVH
+        (False, default).  Speed only; the images are the same."""
+        self.check(self.lib.aurora_set_render_tree(self.h, 1 if wide else 0))
+
     def set_ray_sort(self, mode=1):
         """Ray queries: 0 trace in the given order, 1 sort large queries on
         large trees (default), 2 always sort.  Hits are the same."""

@@ -38,6 +38,7 @@ struct aurora_ctx {
     uint32_t tail_paths = 0xffffffffu;  // fused-tail switch: live paths at or below (default: always)
     uint32_t graph_tail_depth = 8;   // captured passes switch to the fused tail here
     int ray_sort = 1;                // ray queries: 0 never sort, 1 large scenes, 2 always
+    uint32_t render_wide = 0;        // render bounces over the four-wide tree (else binary)
     Workspace ws;                    // path queues shared by the context's renders
     SceneRec* host_recs = nullptr;   // pinned staging of the pass's scene table
     uint32_t host_recs_cap = 0;
@@ -503,6 +504,7 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         }
         pa.recs = w.recs.as<SceneRec>();
         pa.n_recs = (uint32_t)jobs.size();
+        pa.wide = c->render_wide;
         pa.n_paths = path_off;
         pa.n_pixels = pix_off;
         same_shape = sig == prev_sig;
@@ -534,6 +536,7 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         a.hit = w.hit.as<float2>();
         a.recs = w.recs.as<SceneRec>();
         a.one_scene = jobs.size() == 1;
+        a.wide = c->render_wide;  // the counting runs trace as the timed ones do
         a.totals = count_fetches ? jobs[0].scene->totals.as<unsigned long long>() : nullptr;
         a.spill = w.spill.as<uint32_t>();
         a.spill_stride = (uint32_t)(std::max(ext_grid, graph ? graph_tail_grid : tail_grid) * extend_block());
@@ -757,6 +760,13 @@ int aurora_set_launch(aurora_ctx* c, int ext_blocks_per_cu, int shade_blocks_per
     c->ext_grid = c->cus * ext_blocks_per_cu;
     c->shade_grid = c->cus * shade_blocks_per_cu;
     c->tail_paths = tail_paths;
+    return AURORA_OK;
+}
+
+int aurora_set_render_tree(aurora_ctx* c, int wide) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (wide != 0 && wide != 1) return fail(c, AURORA_ERR_INVALID, "render tree must be 0 (binary) or 1 (four-wide)");
+    c->render_wide = (uint32_t)wide;
     return AURORA_OK;
 }
 

@@ -139,6 +139,7 @@ struct Pass {
     uint32_t n_paths;      // sum of WH * spp_pass
     uint32_t n_pixels;     // sum of WH over scenes with spp_pass > 0
     uint32_t any_aov;
+    uint32_t wide;         // render traversal over the four-wide tree (else the binary one)
 };
 struct Workspace {
     DevBuf q_o[2], q_d[2], q_b[2];   // path queues (float4 each)

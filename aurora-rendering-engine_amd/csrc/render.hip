@@ -301,8 +301,10 @@ constexpr int kTailRefill = 4;
 constexpr uint32_t kTailBatch = 64;
 
 // five waves per SIMD: the register budget this leaves (84) holds the kernel
-// without spills; the resident grid (five blocks per CU) matches it
-template <bool ONE>
+// without spills; the resident grid (five blocks per CU) matches it.  WIDE:
+// traverse the four-wide tree, else the binary one (the faster form depends
+// on the scene, so the launch shape chooses).
+template <bool ONE, bool WIDE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_tail(
     Pass pa, ShadeBufs b, uint32_t depth0, uint32_t* spill,
                                                  uint32_t spill_stride) {
@@ -362,8 +364,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         ++segs;
         float t;
         int prim;
-        traverse_closest_binary(r.sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill, spill_stride, t,
-                         prim);
+        if constexpr (WIDE)
+            traverse_closest_wide(r.sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill,
+                                  spill_stride, t, prim);
+        else
+            traverse_closest_binary(r.sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill,
+                                    spill_stride, t, prim);
         const ShadeOut o = shade_core(r, b, org, dir, beta, t, prim, path, depth);
         if (!o.alive) {
             has = false;
@@ -460,7 +466,8 @@ Status launch_tail(const Pass& p, Workspace& ws, int cur, uint32_t depth, int gr
     b.no = b.nd = b.nb = nullptr;
     const size_t need = (size_t)grid * kBlock * (kTravMaxStack - kTravLds) * 4;
     if (need > ws.spill.bytes) return Status{AURORA_ERR_RUNTIME, "tail grid exceeds the stack spill buffer"};
-    auto k = p.n_recs == 1 ? k_tail<true> : k_tail<false>;
+    auto k = p.n_recs == 1 ? (p.wide ? k_tail<true, true> : k_tail<true, false>)
+                           : (p.wide ? k_tail<false, true> : k_tail<false, false>);
     k<<<grid, kBlock, 0, s>>>(p, b, depth, ws.spill.as<uint32_t>(), (uint32_t)(grid * kBlock));
     AUR_HIP(hipGetLastError());
     return Status::ok();

@@ -49,10 +49,29 @@ __device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3
     bool e1 = f1 != AURORA_BVH4_EMPTY && slab(l1, h1, o, inv, tmin, tcur, t1);
     bool e2 = f2 != AURORA_BVH4_EMPTY && slab(l2, h2, o, inv, tmin, tcur, t2);
     bool e3 = f3 != AURORA_BVH4_EMPTY && slab(l3, h3, o, inv, tmin, tcur, t3);
-    if (e0 && (f0 & AURORA_NODE_LEAF)) { test_prim(f0 & ~AURORA_NODE_LEAF); e0 = false; }
-    if (e1 && (f1 & AURORA_NODE_LEAF)) { test_prim(f1 & ~AURORA_NODE_LEAF); e1 = false; }
-    if (e2 && (f2 & AURORA_NODE_LEAF)) { test_prim(f2 & ~AURORA_NODE_LEAF); e2 = false; }
-    if (e3 && (f3 & AURORA_NODE_LEAF)) { test_prim(f3 & ~AURORA_NODE_LEAF); e3 = false; }
+    // the hit leaf slots, compacted in slot order: the wave runs the j-th
+    // primitive test only when some lane has more than j leaves, instead of
+    // once per slot that any lane hit
+    uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+    int np = 0;
+    auto add = [&](bool& e, uint32_t f) {
+        if (!(e && (f & AURORA_NODE_LEAF))) return;
+        const uint32_t id = f & ~AURORA_NODE_LEAF;
+        p0 = np == 0 ? id : p0;
+        p1 = np == 1 ? id : p1;
+        p2 = np == 2 ? id : p2;
+        p3 = np == 3 ? id : p3;
+        ++np;
+        e = false;
+    };
+    add(e0, f0);
+    add(e1, f1);
+    add(e2, f2);
+    add(e3, f3);
+    if (np > 0) test_prim(p0);
+    if (np > 1) test_prim(p1);
+    if (np > 2) test_prim(p2);
+    if (np > 3) test_prim(p3);
     uint64_t k0 = e0 ? wide_key(t0, 0) : ~0ull, k1 = e1 ? wide_key(t1, 1) : ~0ull;
     uint64_t k2 = e2 ? wide_key(t2, 2) : ~0ull, k3 = e3 ? wide_key(t3, 3) : ~0ull;
     wide_cs(k0, k1);
@@ -69,6 +88,65 @@ __device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3
     r2 = ref_of(k2);
     r3 = ref_of(k3);
     return (int)e0 + (int)e1 + (int)e2 + (int)e3;
+}
+
+// Single-ray form of the four-wide traversal, for kernels that trace a ray
+// to completion in one thread.  stk: this thread's LDS column (level k at
+// stk[k * col_stride]); spill: its global column for levels >= kTravLds.
+__device__ __forceinline__ void traverse_closest_wide(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
+                                                      uint32_t* stk, int col_stride, uint32_t* spill, size_t sstride,
+                                                      float& t_out, int& prim_out) {
+    const float4* const W = sc.wide;
+    const float4* const S = sc.spheres;
+    const float4* const Qd = sc.quads;
+    const uint32_t n_spheres = sc.n_spheres, n_wide = sc.n_wide;
+    const uint32_t n_prims = sc.n_spheres + sc.n_quads;
+    const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float best_t = tmax;
+    uint32_t best_id = 0x7fffffffu;
+    auto test_prim = [&](uint32_t id) {
+        if (id >= n_prims) return;
+        float t;
+        bool h;
+        if (id < n_spheres) {
+            h = hit_sphere(ldg4(S, id), o, d, tmin, tmax, t);
+        } else {
+            const size_t q = 3 * (size_t)(id - n_spheres);
+            float al, be;
+            h = hit_quad(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t, al, be);
+        }
+        if (h && (t < best_t || (t == best_t && id < best_id))) {
+            best_t = t;
+            best_id = id;
+        }
+    };
+    uint32_t cur = 0;
+    int sp = 0;
+    auto push = [&](uint32_t v) {
+        if (sp < kTravLds) stk[sp * col_stride] = v;
+        else if (sp < kTravMaxStack) spill[(size_t)(sp - kTravLds) * sstride] = v;
+        if (sp < kTravMaxStack) ++sp;
+    };
+    // a well-formed tree only refers to nodes below n_wide; any other value
+    // ends the ray instead of reading out of bounds
+    while (cur < n_wide) {
+        uint32_t r0, r1, r2, r3;
+        const int m = wide_step(W, cur, o, inv, tmin, best_t, test_prim, r0, r1, r2, r3);
+        if (m == 0) {
+            if (sp == 0) break;
+            --sp;
+            if (sp < kTravLds) cur = stk[sp * col_stride];
+            else cur = ldg1(spill, (size_t)(sp - kTravLds) * sstride);
+            continue;
+        }
+        if (m > 3) push(r3);
+        if (m > 2) push(r2);
+        if (m > 1) push(r1);
+        cur = r0;
+    }
+    const bool miss = best_id == 0x7fffffffu;
+    t_out = miss ? tmax : best_t;
+    prim_out = miss ? AURORA_MISS : (int)best_id;
 }
 
 // Binary form: one step fetches the aligned child pair at `cur` (two node

@@ -84,12 +84,15 @@ def stress_section(ctx, n_rays=1 << 22, render_spp=1):
     cam = scenes.stress_camera(1920, 1080)
     rgb = torch.zeros((1080, 1920, 3), dtype=torch.float32, device="cuda")
     rs = RenderSettings(spp=render_spp, max_depth=16, seed=0, background=(0.7, 0.8, 1.0), tmin=1e-4)
+    ctx.set_render_tree(False)
     rc = scene.render_device(cam, RenderSettings(spp=render_spp, max_depth=16, seed=0, background=(0.7, 0.8, 1.0),
                                                  tmin=1e-4, count_fetches=True), rgb)
     ralg = 32 * rc["nodes_fetched"] + 16 * rc["spheres_fetched"] + 48 * rc["quads_fetched"]
-    render = {"alg_traversal_bytes": ralg}
-    for mode, tp in (("wavefront_tail_1M", 1 << 20), ("persistent_tail", TAIL_ALWAYS)):
+    render = {"alg_traversal_bytes_binary": ralg}
+    for mode, tp, wide in (("wavefront_tail_1M", 1 << 20, False), ("persistent_tail", TAIL_ALWAYS, False),
+                           ("persistent_tail_four_wide", TAIL_ALWAYS, True)):
         ctx.set_launch(tail_paths=tp)
+        ctx.set_render_tree(wide)
         scene.render_device(cam, rs, rgb)
         ctx.synchronize()
         t1 = time.perf_counter()
@@ -98,10 +101,12 @@ def stress_section(ctx, n_rays=1 << 22, render_spp=1):
         wall = time.perf_counter() - t1
         k_ms = st["extend_ms"] + st["tail_ms"]
         render[mode] = {"paths": st["paths"], "paths_per_s": st["paths"] / wall, "rays_per_s": st["segments"] / wall,
-                        "extend_ms": st["extend_ms"], "tail_ms": st["tail_ms"],
-                        "traversal_kernels_achieved_GBs": ralg / (k_ms * 1e-3) / 1e9}
+                        "extend_ms": st["extend_ms"], "tail_ms": st["tail_ms"]}
+        if not wide:
+            render[mode]["traversal_kernels_achieved_GBs"] = ralg / (k_ms * 1e-3) / 1e9
         note(f"stress render {mode}: {st['paths'] / wall / 1e6:.1f} Mpaths/s")
     ctx.set_launch()
+    ctx.set_render_tree(False)
     scene.close()
     return {"workload": "stress_2M_spheres_2M_triangles", "bvh_nodes": bvh["n_nodes"], "bvh_depth": bvh["depth"],
             "bvh_build_ms": bvh["build_ms"], "traversal": trav, "traversal_unsorted": trav_unsorted,
@@ -114,9 +119,9 @@ TAIL_ALWAYS = (1 << 32) - 1  # fused-tail switch above any queue: one persistent
 
 # HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain 1 GiB copy
-# gives), keyed (kernel, width, tail switch); the summaries and commands
-# are in the profiles/ directory.
-PMC_TRAFFIC = {("k_tail", 1024, TAIL_ALWAYS): 668.4e6}  # profiles/tail_cornell_1024x1024.md
+# gives), keyed (kernel, width, tail switch, four-wide render tree); the
+# summaries and commands are in the profiles/ directory.
+PMC_TRAFFIC = {}  # profiles/tail_cornell_1024x1024.md
 
 
 def note(msg):
@@ -260,7 +265,8 @@ def serving_section(ctx, n_scenes=64, seed0=0):
     note(f"serving: {n_scenes} scenes uploaded ({setup:.1f} s); autotuning")
     best, table = autotune.tune(ctx, run, autotune.signature(items), cache_path=None, log=note,
                                 candidates={"extend_blocks_per_cu": [8, 16], "shade_blocks_per_cu": [8],
-                                            "tail_paths": [1 << 20, TAIL_ALWAYS], "graph": [False, True]})
+                                            "tail_paths": [1 << 20, TAIL_ALWAYS], "graph": [False, True],
+                                            "wide": [False, True]})
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     st = run(best["graph"])
@@ -303,6 +309,8 @@ def main():
     ap.add_argument("--no-metric", action="store_true", help="skip time-to-relMSE (plain vs denoised)")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving section")
     ap.add_argument("--no-denoiser", action="store_true", help="skip the denoiser throughput section")
+    ap.add_argument("--render-tree", choices=["binary", "wide"], default="binary",
+                    help="render traversal tree when --tail-paths fixes the shape (no autotuning)")
     ap.add_argument("--only", choices=["stress", "copy", "serving", "denoiser"], default=None,
                     help="profiling modes: the stress section alone, or a plain 1 GiB device copy "
                          "(calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -368,11 +376,12 @@ def main():
         best, tune_table = autotune.tune(ctx, tune_run, f"cornell_{W}x{H}_spp{S}", cache_path=None, log=note,
                                          candidates={"extend_blocks_per_cu": [8], "shade_blocks_per_cu": [8],
                                                      "tail_paths": [1 << 20, TAIL_ALWAYS],
-                                                     "graph": [False, True]})
+                                                     "graph": [False, True], "wide": [False, True]})
         graph = best["graph"]
         shape = {k: best[k] for k in best if k != "seconds"}
     else:
-        shape = {"tail_paths": args.tail_paths}
+        ctx.set_render_tree(args.render_tree == "wide")
+        shape = {"tail_paths": args.tail_paths, "wide": args.render_tree == "wide"}
 
     if args.sweep_tail:
         for tp in (0, 262144, 1 << 20, 4 << 20, TAIL_ALWAYS):
@@ -425,7 +434,7 @@ def main():
         kernel, launches, k_ms = "k_extend", max(1, first["extend_launches"]), first["extend_ms"]
         alg_bytes = alg_trav
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-    traffic = PMC_TRAFFIC.get((kernel, W, shape.get("tail_paths")))
+    traffic = PMC_TRAFFIC.get((kernel, W, shape.get("tail_paths"), bool(shape.get("wide"))))
     out = {
         "metric": METRIC,
         "value": paths / elapsed,

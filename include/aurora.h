@@ -224,6 +224,12 @@ void* aurora_stream(aurora_ctx* ctx);
 int aurora_set_launch(aurora_ctx* ctx, int extend_blocks_per_cu,
                       int shade_blocks_per_cu, uint32_t tail_paths);
 
+/* Tree walked by the render kernels: 0 = the binary LBVH (default; one
+ * 64-byte child pair per step), 1 = the four-wide tree (one 128-byte node
+ * per step).  Which is faster depends on the scene (the autotuner tries
+ * both); the images are the same.  No reference counterpart. */
+int aurora_set_render_tree(aurora_ctx* ctx, int wide);
+
 /* Order of the rays of aurora_trace_closest*: 0 = as given, 1 = sorted by
  * origin (Morton code in the root box) and direction octant when the query
  * has at least 2^20 rays and the tree at least 2^21 nodes (default), 2 =

@@ -9,9 +9,13 @@ from aurora_amd import autotune
 class FakeCtx:
     def __init__(self):
         self.shape = None
+        self.wide = None
 
     def set_launch(self, e, s, t):
         self.shape = (e, s, t)
+
+    def set_render_tree(self, wide):
+        self.wide = wide
 
 
 def test_picks_cheapest_and_caches(tmp_path, monkeypatch):
@@ -19,20 +23,20 @@ def test_picks_cheapest_and_caches(tmp_path, monkeypatch):
     calls = []
     clock = [0.0]
 
-    def cost(shape, graph):
+    def cost(shape, graph, wide):
         e, s, t = shape
-        return 1.0 + abs(e - 8) + (0.5 if t != 1 << 20 else 0.0) + (0.0 if graph else 0.25)
+        return 1.0 + abs(e - 8) + (0.5 if t != 1 << 20 else 0.0) + (0.0 if graph else 0.25) + (0.1 if wide else 0.0)
 
     def run(graph):
         calls.append((ctx.shape, graph))
-        clock[0] += cost(ctx.shape, graph)
+        clock[0] += cost(ctx.shape, graph, ctx.wide)
 
     monkeypatch.setattr(autotune.time, "perf_counter", lambda: clock[0])
     cache = tmp_path / "tune.json"
     best, table = autotune.tune(ctx, run, "sigA", cache_path=str(cache))
     assert (best["extend_blocks_per_cu"], best["tail_paths"], best["graph"]) == (8, 1 << 20, True)
-    assert ctx.shape == (8, 8, 1 << 20)
-    n_cand = 3 * 1 * 2 * 2
+    assert ctx.shape == (8, 8, 1 << 20) and ctx.wide is False
+    n_cand = 3 * 1 * 2 * 2 * 2
     assert len(table) == n_cand and len(calls) == n_cand * 3  # warm-up + 2 repeats each
     saved = json.loads(cache.read_text())
     assert saved["sigA"]["best"]["graph"] is True

@@ -52,13 +52,14 @@ def test_batch_equals_single_renders(ctx):
             st = sc.render_device(cam, rs, rgb, alb if rs.aov else None, nrm if rs.aov else None)
             singles.append((rgb.cpu().numpy(), alb.cpu().numpy() if rs.aov else None))
             seg += st["segments"]
-        for tail in TAILS:
+        for tail, wide in [(t, False) for t in TAILS] + [(TAILS[-1], True)]:
             ctx.set_launch(tail_paths=tail)
+            ctx.set_render_tree(wide)
             for graph in (False, True):
                 bufs = [(zeros(cam), zeros(cam), zeros(cam)) for _, cam, _ in items]
                 st = render_batch(ctx, [(sc, cam, rs, b[0], b[1] if rs.aov else None, b[2] if rs.aov else None)
                                         for (sc, cam, rs), b in zip(items, bufs)], graph=graph)
-                print("tail", tail, "graph", graph, "paths", st["paths"], "segments", st["segments"],
+                print("tail", tail, "wide", wide, "graph", graph, "paths", st["paths"], "segments", st["segments"],
                       "single", seg, flush=True)
                 assert st["segments"] == seg
                 for k, ((rgb, alb, _), (s_rgb, s_alb)) in enumerate(zip(bufs, singles)):
@@ -67,6 +68,7 @@ def test_batch_equals_single_renders(ctx):
                         assert np.array_equal(alb.cpu().numpy(), s_alb), (tail, graph, k)
     finally:
         ctx.set_launch()
+        ctx.set_render_tree(False)
 
 
 @pytest.mark.parametrize("tail", TAILS)

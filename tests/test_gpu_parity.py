@@ -210,15 +210,18 @@ def test_render_radiance_bit_exact(ctx, name, mk, mkcam, rs):
     cam = mkcam()
     g, gs, o, os_ = both(ctx, arr)
     orgb, _, _, segs = o.render(cam, rs)
-    # pure wavefront, wavefront then fused tail, fused from the first bounce
-    for tail in (0, 2048, 1 << 30):
+    # pure wavefront, wavefront then fused tail, fused from the first bounce;
+    # binary and four-wide render trees
+    for tail, wide in ((0, False), (2048, False), (1 << 30, False), (0, True), (1 << 30, True)):
         ctx.set_launch(tail_paths=tail)
+        ctx.set_render_tree(wide)
         try:
             grgb, _, _, st = g.render(cam, rs)
         finally:
             ctx.set_launch()
+            ctx.set_render_tree(False)
         eq = (grgb.view(np.uint32) == orgb.view(np.uint32)).all(-1)
-        say(name, "tail", tail, "pixels bit-equal:", eq.mean(), "segments gpu/cpu", st["segments"], segs)
+        say(name, "tail", tail, "wide", wide, "pixels bit-equal:", eq.mean(), "segments gpu/cpu", st["segments"], segs)
         assert st["paths"] == cam.image_width * cam.image_height * rs.spp
         assert st["segments"] == segs
         assert eq.all()
