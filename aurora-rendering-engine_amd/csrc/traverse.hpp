@@ -17,24 +17,26 @@ namespace aur {
 constexpr int kTravLds = 16;       // stack levels kept in LDS
 constexpr int kTravMaxStack = 96;  // a four-wide step pushes at most 3 per level (depth <= 64)
 
-// Sort key of a hit internal slot: entry distance (ordered bits; -0 taken
-// as +0 so that the order is the float order), then slot index.
-__device__ __forceinline__ uint64_t wide_key(float t, uint32_t slot) {
-    uint32_t u = __float_as_uint(t + 0.0f);
-    u ^= (u & 0x80000000u) ? 0xffffffffu : 0x80000000u;
-    return ((uint64_t)u << 2) | slot;
-}
-__device__ __forceinline__ void wide_cs(uint64_t& a, uint64_t& b) {
-    const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
-    a = lo;
-    b = hi;
+// Compare-exchange of (key, ref) pairs, swapping only when the second key
+// is strictly smaller.  Keys are the unsigned bits of entry t + 0 (-0 taken
+// as +0): for t >= +0 that is the float order, and one 32-bit compare orders
+// the pair and carries the ref along.  Slots that are not hit are keyed all
+// ones and sort last.
+__device__ __forceinline__ void wide_cs(uint32_t& ka, uint32_t& ra, uint32_t& kb, uint32_t& rb) {
+    const bool sw = kb < ka;
+    const uint32_t k = sw ? kb : ka, r = sw ? rb : ra;
+    kb = sw ? ka : kb;
+    rb = sw ? ra : rb;
+    ka = k;
+    ra = r;
 }
 
 // One step of the four-wide traversal at node `cur` (8 float4: per slot
 // (lo, ref), (hi, 0)): slab tests of the four slots against tcur, then the
 // hit leaf slots' primitives in slot order (test_prim), then the hit
-// internal slots ordered by entry distance, ties by slot.  Returns their
-// count; ref[0..count) holds them nearest first.
+// internal slots nearest first by the comparators (0,1) (2,3) (0,2) (1,3)
+// (1,2) (equal distances stay where the network leaves them).  Returns
+// their count; ref[0..count) holds them in that order.
 template <class TestPrim>
 __device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3 inv, float tmin, float tcur,
                                          TestPrim&& test_prim, uint32_t& r0, uint32_t& r1, uint32_t& r2,
@@ -72,21 +74,17 @@ __device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3
     if (np > 1) test_prim(p1);
     if (np > 2) test_prim(p2);
     if (np > 3) test_prim(p3);
-    uint64_t k0 = e0 ? wide_key(t0, 0) : ~0ull, k1 = e1 ? wide_key(t1, 1) : ~0ull;
-    uint64_t k2 = e2 ? wide_key(t2, 2) : ~0ull, k3 = e3 ? wide_key(t3, 3) : ~0ull;
-    wide_cs(k0, k1);
-    wide_cs(k2, k3);
-    wide_cs(k0, k2);
-    wide_cs(k1, k3);
-    wide_cs(k1, k2);
-    auto ref_of = [&](uint64_t k) {
-        const uint32_t sl = (uint32_t)k & 3u;
-        return sl == 0 ? f0 : sl == 1 ? f1 : sl == 2 ? f2 : f3;
-    };
-    r0 = ref_of(k0);
-    r1 = ref_of(k1);
-    r2 = ref_of(k2);
-    r3 = ref_of(k3);
+    uint32_t k0 = e0 ? __float_as_uint(t0 + 0.0f) : ~0u, k1 = e1 ? __float_as_uint(t1 + 0.0f) : ~0u;
+    uint32_t k2 = e2 ? __float_as_uint(t2 + 0.0f) : ~0u, k3 = e3 ? __float_as_uint(t3 + 0.0f) : ~0u;
+    r0 = f0;
+    r1 = f1;
+    r2 = f2;
+    r3 = f3;
+    wide_cs(k0, r0, k1, r1);
+    wide_cs(k2, r2, k3, r3);
+    wide_cs(k0, r0, k2, r2);
+    wide_cs(k1, r1, k3, r3);
+    wide_cs(k1, r1, k2, r2);
     return (int)e0 + (int)e1 + (int)e2 + (int)e3;
 }
 

@@ -41,6 +41,10 @@ void orc_prim_box(orc_scene* s, uint32_t id, float* lo, float* hi);
  * is 4 node records), tree modes only. */
 int orc_trace(orc_scene* s, const float* o, const float* d, uint32_t n, float tmin, float tmax, int brute,
               int32_t* prim, float* t, uint32_t* counts, int threads);
+/* the order in which a four-wide step descends into its hit internal slots:
+ * t[4] entry distances, hit[4] flags; ord[0..m) receives the hit slots
+ * nearest first, the rest the others; returns m */
+int orc_wide_order(const float* t, const int32_t* hit, int32_t* ord);
 int orc_hit_sphere(const float* sphere, const float* o, const float* d, float tmin, float tmax, float* t);
 /* tab receives t, alpha, beta */
 int orc_hit_quad(const float* quad, uint32_t kind, const float* o, const float* d, float tmin, float tmax,

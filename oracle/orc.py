@@ -43,6 +43,7 @@ def lib():
             "orc_bvh_wide": (C.c_int, [vp, vp, u32]),
             "orc_prim_box": (None, [vp, u32, vp, vp]),
             "orc_trace": (C.c_int, [vp, vp, vp, u32, f, f, C.c_int, vp, vp, vp, C.c_int]),
+            "orc_wide_order": (C.c_int, [vp, vp, vp]),
             "orc_hit_sphere": (C.c_int, [vp, vp, vp, f, f, vp]),
             "orc_hit_quad": (C.c_int, [vp, u32, vp, vp, f, f, vp]),
             "orc_texture_value": (None, [vp, u32, f, f, vp, vp]),
@@ -116,6 +117,16 @@ def random_unit(pixel, sample, bounce, seed=0):
     out = np.zeros(3, np.float32)
     lib().orc_random_unit(pixel, sample, bounce, seed, out.ctypes.data)
     return out
+
+
+def wide_order(t, hit):
+    """Descent order of a four-wide step: (m, ord) with ord[:m] the hit
+    slots nearest first."""
+    tt = np.ascontiguousarray(t, np.float32)
+    hh = np.ascontiguousarray(hit, np.int32)
+    out = np.zeros(4, np.int32)
+    m = lib().orc_wide_order(tt.ctypes.data, hh.ctypes.data, out.ctypes.data)
+    return m, out
 
 
 def camera_frame(cam):

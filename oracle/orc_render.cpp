@@ -421,6 +421,15 @@ int orc_trace(orc_scene* sc, const float* o, const float* d, uint32_t n, float t
     return AURORA_OK;
 }
 
+int orc_wide_order(const float* t, const int32_t* hit, int32_t* ord) {
+    bool h[4];
+    int o[4];
+    for (int k = 0; k < 4; ++k) h[k] = hit[k] != 0;
+    const int m = wide_order<float>(h, t, o);
+    for (int k = 0; k < 4; ++k) ord[k] = o[k];
+    return m;
+}
+
 int orc_hit_sphere(const float* sphere, const float* o, const float* d, float tmin, float tmax, float* t) {
     Sphere sp{{sphere[0], sphere[1], sphere[2]}, sphere[3]};
     return hit_sphere<float>(sp, mk(o[0], o[1], o[2]), mk(d[0], d[1], d[2]), tmin, tmax, *t) ? 1 : 0;

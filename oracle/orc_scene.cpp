@@ -525,10 +525,52 @@ static inline bool slab4(const aurora_bvh4_slot& sl, V3<R> o, V3<R> inv, R tmin,
     return slab(nd, o, inv, tmin, tcur, tnear);
 }
 
+// Order key of a slot's entry distance: the unsigned bits of t + 0 (-0
+// becomes +0), which for t >= +0 is the numeric order; a negative t (only
+// when tmin < 0) sorts after every non-negative one.  Slots that are not hit
+// are keyed all ones, so they sort last.
+static inline uint32_t wide_order_bits(float t) {
+    const float u = t + 0.0f;
+    uint32_t b;
+    std::memcpy(&b, &u, 4);
+    return b;
+}
+static inline uint64_t wide_order_bits(double t) {
+    const double u = t + 0.0;
+    uint64_t b;
+    std::memcpy(&b, &u, 8);
+    return b;
+}
+
+// Nearest-first order of a four-wide node's hit internal slots, as the GPU
+// step computes it (aurora traverse.hpp wide_step): the five comparators
+// (0,1) (2,3) (0,2) (1,3) (1,2) on (key, slot) pairs, each swapping only
+// when the second key is strictly smaller.  Equal keys stay where the network
+// leaves them.  ord[0..m) receives the hit slots; returns m.
+template <class R>
+int wide_order(const bool hit[4], const R tn[4], int ord[4]) {
+    using K = decltype(wide_order_bits(R(0)));
+    K key[4];
+    int m = 0;
+    for (int k = 0; k < 4; ++k) {
+        key[k] = hit[k] ? wide_order_bits(tn[k]) : ~K(0);
+        ord[k] = k;
+        m += hit[k] ? 1 : 0;
+    }
+    static const int net[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+    for (const auto& c : net)
+        if (key[c[1]] < key[c[0]]) {
+            std::swap(key[c[0]], key[c[1]]);
+            std::swap(ord[c[0]], ord[c[1]]);
+        }
+    return m;
+}
+template int wide_order<float>(const bool[4], const float[4], int[4]);
+
 // One step tests the four slots against the current best t, tests the hit
 // leaf slots' primitives in slot order, then orders the hit internal slots
-// by entry distance (ties: slot order), descends to the nearest and pushes
-// the others farthest first.
+// nearest first (wide_order), descends to the nearest and pushes the others
+// farthest first.
 template <class R>
 Hit<R> closest_bvh4(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt) {
     Hit<R> best{tmax, 0x7fffffff};
@@ -553,12 +595,8 @@ Hit<R> closest_bvh4(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cn
                 test_prim(s, w.slot[k].ref & ~AURORA_NODE_LEAF, o, d, tmin, tmax, best, cnt);
                 hit[k] = false;
             }
-        // stable order of the hit internal slots by entry distance
-        int ord[4], m = 0;
-        for (int k = 0; k < 4; ++k)
-            if (hit[k]) ord[m++] = k;
-        for (int a = 1; a < m; ++a)
-            for (int b = a; b > 0 && tn[ord[b]] < tn[ord[b - 1]]; --b) std::swap(ord[b], ord[b - 1]);
+        int ord[4];
+        const int m = wide_order<R>(hit, tn, ord);
         if (m == 0) {
             if (sp == 0) break;
             cur = stack[--sp];

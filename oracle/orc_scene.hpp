@@ -63,6 +63,8 @@ Hit<R> closest_bvh(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt
 template <class R>
 Hit<R> closest_bvh4(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt);
 template <class R>
+int wide_order(const bool hit[4], const R tn[4], int ord[4]);
+template <class R>
 Hit<R> closest_brute(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax);
 
 // single-primitive tests; return true and t when hit in (tmin, tmax)

@@ -205,3 +205,24 @@ def test_bvh_equals_brute_force(name, arr):
     assert np.all(cnt[pb >= 0].sum(1) > 1)
     if ns == 0:
         assert np.all(cnt[:, 1] == 0)
+
+
+def test_wide_descent_order_sorts():
+    """The four-wide step's descent order is a sorting network: for every
+    hit pattern and every assignment of entry distances from a set with
+    ties and both zeros, the hit slots come first, each once, nearest first
+    (all 4^4 x 2^4 inputs; by the 0-1 principle a network that sorts these
+    sorts everything)."""
+    import itertools
+    vals = [np.float32(-0.0), np.float32(0.0), np.float32(0.5), np.float32(2.0)]
+    for hit in itertools.product([0, 1], repeat=4):
+        for t in itertools.product(vals, repeat=4):
+            m, ordv = orc.wide_order(t, hit)
+            assert m == sum(hit)
+            assert sorted(ordv.tolist()) == [0, 1, 2, 3]
+            assert sorted(ordv[:m].tolist()) == [k for k in range(4) if hit[k]]
+            ts = [float(t[k]) for k in ordv[:m]]
+            assert ts == sorted(ts), (hit, t, ordv)
+    # a negative entry distance (tmin < 0) sorts after the non-negative ones
+    m, ordv = orc.wide_order([-1.0, 3.0, 0.0, 1.0], [1, 1, 1, 1])
+    assert m == 4 and ordv.tolist() == [2, 3, 1, 0]
