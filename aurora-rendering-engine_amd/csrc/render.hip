@@ -313,10 +313,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     uint32_t* my_spill = spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // the queue is split into one contiguous range per XCD; a wave whose
+    // range is spent moves on to the next ones, so uneven ranges (a batch's
+    // scenes differ in cost per path) still finish together
     const uint32_t parts = min((uint32_t)kPools, gridDim.x);
-    const uint32_t part = blockIdx.x % parts;
-    const uint32_t r_lo = (uint32_t)(((uint64_t)n * part) / parts);
-    const uint32_t r_n = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - r_lo;
+    uint32_t part = blockIdx.x % parts, spent = 0;
+    uint32_t r_lo = (uint32_t)(((uint64_t)n * part) / parts);
+    uint32_t r_n = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - r_lo;
     uint32_t* pool = b.counters + kPoolBase + kPoolStride * part;
     uint32_t b_next = 0, b_end = 0;  // wave-uniform, offsets within the range
     bool has = false, done = false;
@@ -328,17 +331,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         const uint64_t im = __ballot(idle);
         const uint64_t am = __ballot(has);
         if (im != 0 && (am == 0 || __popcll(im) >= kTailRefill)) {
-            if (b_next >= b_end) {
+            // at most `parts` ranges are tried, each once after it is spent
+            while (b_next >= b_end && spent < parts) {
                 uint32_t base = r_n;
                 if (lane == 0) {
                     if (__hip_atomic_load(pool, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < r_n)
                         base = atomicAdd(pool, kTailBatch);
                 }
                 base = __shfl(base, 0);
-                b_next = base;
-                b_end = base < r_n ? min(r_n, base + kTailBatch) : base;
+                if (base < r_n) {
+                    b_next = base;
+                    b_end = min(r_n, base + kTailBatch);
+                } else if (++spent < parts) {
+                    part = part + 1 == parts ? 0u : part + 1;
+                    r_lo = (uint32_t)(((uint64_t)n * part) / parts);
+                    r_n = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - r_lo;
+                    pool = b.counters + kPoolBase + kPoolStride * part;
+                }
             }
-            const uint32_t avail = b_end - b_next;
+            const uint32_t avail = b_next < b_end ? b_end - b_next : 0u;
             const uint32_t rank = (uint32_t)__popcll(im & lt);
             if (idle) {
                 if (avail == 0) {

@@ -113,10 +113,12 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
     // with its own pool counter on its own 128-byte line.  A wave claims a
     // batch of its range with one atomic and deals it out to idle lanes by
     // ballot rank; a plain load skips the atomic once the range is spent.
+    // A wave whose range is spent moves on to the next ones, so uneven
+    // ranges still finish together.
     const uint32_t parts = min((uint32_t)kParts, gridDim.x);
-    const uint32_t part = blockIdx.x % parts;
-    const uint32_t r_lo = (uint32_t)(((uint64_t)n * part) / parts);
-    const uint32_t r_n = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - r_lo;
+    uint32_t part = blockIdx.x % parts, spent = 0;
+    uint32_t r_lo = (uint32_t)(((uint64_t)n * part) / parts);
+    uint32_t r_n = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - r_lo;
     uint32_t* pool = a.pool + kPoolStride * part;
     const uint32_t waves = ((gridDim.x - part + parts - 1) / parts) * (kBlock / 64);
     const uint32_t batch = min(4096u, max(64u, r_n / (waves * 2u) + 1u));
@@ -126,17 +128,25 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
         const uint64_t im = __ballot(idle);
         const uint64_t am = __ballot(ri != kNone);
         if (im != 0 && (am == 0 || __popcll(im) >= kRefill)) {
-            if (b_next >= b_end) {
+            // at most `parts` ranges are tried, each once after it is spent
+            while (b_next >= b_end && spent < parts) {
                 uint32_t base = r_n;
                 if (lane == 0) {
                     if (__hip_atomic_load(pool, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < r_n)
                         base = atomicAdd(pool, batch);
                 }
                 base = __shfl(base, 0);
-                b_next = base;
-                b_end = base < r_n ? min(r_n, base + batch) : base;
+                if (base < r_n) {
+                    b_next = base;
+                    b_end = min(r_n, base + batch);
+                } else if (++spent < parts) {
+                    part = part + 1 == parts ? 0u : part + 1;
+                    r_lo = (uint32_t)(((uint64_t)n * part) / parts);
+                    r_n = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - r_lo;
+                    pool = a.pool + kPoolStride * part;
+                }
             }
-            const uint32_t avail = b_end - b_next;
+            const uint32_t avail = b_next < b_end ? b_end - b_next : 0u;
             const uint32_t rank = (uint32_t)__popcll(im & lt);
             if (idle) {
                 if (avail == 0) {
