@@ -318,6 +318,38 @@ def random_rays(n, lo, hi, seed=0):
     return o.astype(np.float32), g.astype(np.float32)
 
 
+def silhouette_scene(seed=0, n_spheres=3000, n_quads=3000, n_rays=40000):
+    """Small spheres (r in [0.004, 0.02]) and small quads crossing each other
+    in [-0.3, 0.3]^3, seen from 4 units away, and rays aimed at the sphere
+    silhouettes (within 0.5% inside to 0.05% outside the rim).  This is where
+    the textbook ray/sphere quadratic loses its discriminant and where the
+    closest hit would depend on the visit order without the leaf-box rule.
+    Returns (arrays, origins, directions) as float32."""
+    rng = np.random.default_rng(seed)
+    b = Builder()
+    m = b.lambertian((0.5, 0.5, 0.5))
+    c = rng.uniform(-0.3, 0.3, (n_spheres, 3))
+    r = rng.uniform(0.004, 0.02, n_spheres)
+    for k in range(n_spheres):
+        b.sphere(tuple(c[k]), float(r[k]), m)
+    Q = rng.uniform(-0.3, 0.3, (n_quads, 3))
+    u = rng.uniform(-0.03, 0.03, (n_quads, 3))
+    v = rng.uniform(-0.03, 0.03, (n_quads, 3))
+    for k in range(n_quads):
+        b.quad(tuple(Q[k]), tuple(u[k]), tuple(v[k]), m)
+    org = np.array([0.0, 0.2, 4.0])
+    ks = rng.integers(0, n_spheres, n_rays)
+    w = rng.normal(size=(n_rays, 3))
+    to = c[ks] - org
+    to /= np.linalg.norm(to, axis=1, keepdims=True)
+    perp = w - (w * to).sum(1, keepdims=True) * to
+    perp /= np.linalg.norm(perp, axis=1, keepdims=True)
+    tgt = c[ks] + perp * (r[ks] * rng.uniform(0.995, 1.0005, n_rays))[:, None]
+    d = (tgt - org).astype(np.float32)
+    o = np.broadcast_to(org.astype(np.float32), d.shape).copy()
+    return b.arrays(), o, d
+
+
 def random_rays_torch(n, lo, hi, seed=0, device="cuda"):
     """Device-side ray set for timing runs: origins uniform in [lo, hi]^3,
     directions normalised Gaussian triples (torch generator seeded `seed`)."""

@@ -185,17 +185,31 @@ struct DevScene {
     uint32_t n_spheres, n_quads, n_nodes, n_wide;
 };
 
+// Ray/sphere roots without the cancellations of the textbook quadratic
+// (Haines, Guenther and Akenine-Moeller, "Precision Improvements for
+// Ray/Sphere Intersection", Ray Tracing Gems, 2019): the discriminant
+// h^2 - a*c is formed as a*r^2 - |d x oc|^2 (Lagrange's identity), whose
+// terms are of the size of r^2 instead of |oc|^2, and the roots as c/q and
+// q/a with q = h + sign(h)*sqrt(disc), which adds terms of one sign.  The
+// textbook form loses every digit of the discriminant for a small sphere
+// seen from afar and then reports hits outside the sphere.  The near root
+// (c/q when q > 0, else q/a) is taken when it lies in (tmin, tmax), else the
+// far one; each is one division of a selected numerator and denominator.
 __device__ __forceinline__ bool hit_sphere(float4 sp, v3 o, v3 d, float tmin, float tmax, float& t) {
-    v3 oc = V(sp.x, sp.y, sp.z) - o;
-    float a = dot(d, d);
-    float h = dot(d, oc);
-    float c = dot(oc, oc) - sp.w * sp.w;
-    float disc = h * h - a * c;
-    if (disc < 0.0f) return false;
-    float sq = sqrtf(disc);
-    float root = (h - sq) / a;
+    const v3 oc = V(sp.x, sp.y, sp.z) - o;
+    const float a = dot(d, d);
+    const float h = dot(d, oc);
+    const float rr = sp.w * sp.w;
+    const float c = dot(oc, oc) - rr;
+    const v3 x = cross(d, oc);
+    const float disc = a * rr - dot(x, x);
+    if (!(disc >= 0.0f)) return false;
+    const float q = h + copysignf(sqrtf(disc), h);
+    if (q == 0.0f) return false;  // tangent at the origin
+    const bool pos = q > 0.0f;
+    float root = (pos ? c : q) / (pos ? q : a);
     if (!(root > tmin && root < tmax)) {
-        root = (h + sq) / a;
+        root = (pos ? q : c) / (pos ? a : q);
         if (!(root > tmin && root < tmax)) return false;
     }
     t = root;
@@ -236,6 +250,43 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, v3 o, v3 inv, float t
     float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tcur));
     tnear = tn;
     return tn <= tf;
+}
+
+// the box of primitive id as the LBVH stores it in the primitive's leaf
+__device__ __forceinline__ void prim_box(const DevScene& sc, uint32_t id, v3& lo, v3& hi) {
+    if (id < sc.n_spheres) {
+        float4 s = sc.spheres[id];
+        lo = V(s.x - s.w, s.y - s.w, s.z - s.w);
+        hi = V(s.x + s.w, s.y + s.w, s.z + s.w);
+    } else {
+        const float4* q = sc.quads + 3 * (size_t)(id - sc.n_spheres);
+        float4 q0 = q[0], q1 = q[1], q2 = q[2];
+        const bool para = __float_as_uint(q1.w) == AURORA_QUAD_PARALLELOGRAM;
+        float a[3] = {q0.x, q0.y, q0.z}, u[3] = {q1.x, q1.y, q1.z}, v[3] = {q2.x, q2.y, q2.z};
+        float l[3], h[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float p0 = a[k], p1 = a[k] + u[k], p2 = a[k] + v[k];
+            float lk = fminf(p0, fminf(p1, p2)), hk = fmaxf(p0, fmaxf(p1, p2));
+            if (para) {
+                float p3 = (a[k] + u[k]) + v[k];
+                lk = fminf(lk, p3);
+                hk = fmaxf(hk, p3);
+            }
+            l[k] = lk;
+            h[k] = hk;
+        }
+        lo = V(l[0], l[1], l[2]);
+        hi = V(h[0], h[1], h[2]);
+    }
+    // pad by 2^-16 of the largest coordinate magnitude (planar quads get
+    // thickness; slab-test rounding cannot cull a true hit)
+    float m = fmaxf(fmaxf(fabsf(lo.x), fabsf(hi.x)), 0.0f);
+    m = fmaxf(m, fmaxf(fabsf(lo.y), fabsf(hi.y)));
+    m = fmaxf(m, fmaxf(fabsf(lo.z), fabsf(hi.z)));
+    float e = m * (1.0f / 65536.0f) + 1e-30f;
+    lo = V(lo.x - e, lo.y - e, lo.z - e);
+    hi = V(hi.x + e, hi.y + e, hi.z + e);
 }
 
 // ------------------------------------------------------------- textures ---

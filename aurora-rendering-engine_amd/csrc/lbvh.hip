@@ -39,42 +39,6 @@ __host__ __device__ __forceinline__ float o2f(uint32_t o) {
     return c.f;
 }
 
-__device__ __forceinline__ void prim_box(const DevScene& sc, uint32_t id, v3& lo, v3& hi) {
-    if (id < sc.n_spheres) {
-        float4 s = sc.spheres[id];
-        lo = V(s.x - s.w, s.y - s.w, s.z - s.w);
-        hi = V(s.x + s.w, s.y + s.w, s.z + s.w);
-    } else {
-        const float4* q = sc.quads + 3 * (size_t)(id - sc.n_spheres);
-        float4 q0 = q[0], q1 = q[1], q2 = q[2];
-        const bool para = __float_as_uint(q1.w) == AURORA_QUAD_PARALLELOGRAM;
-        float a[3] = {q0.x, q0.y, q0.z}, u[3] = {q1.x, q1.y, q1.z}, v[3] = {q2.x, q2.y, q2.z};
-        float l[3], h[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            float p0 = a[k], p1 = a[k] + u[k], p2 = a[k] + v[k];
-            float lk = fminf(p0, fminf(p1, p2)), hk = fmaxf(p0, fmaxf(p1, p2));
-            if (para) {
-                float p3 = (a[k] + u[k]) + v[k];
-                lk = fminf(lk, p3);
-                hk = fmaxf(hk, p3);
-            }
-            l[k] = lk;
-            h[k] = hk;
-        }
-        lo = V(l[0], l[1], l[2]);
-        hi = V(h[0], h[1], h[2]);
-    }
-    // pad by 2^-16 of the largest coordinate magnitude (planar quads get
-    // thickness; slab-test rounding cannot cull a true hit)
-    float m = fmaxf(fmaxf(fabsf(lo.x), fabsf(hi.x)), 0.0f);
-    m = fmaxf(m, fmaxf(fabsf(lo.y), fabsf(hi.y)));
-    m = fmaxf(m, fmaxf(fabsf(lo.z), fabsf(hi.z)));
-    float e = m * (1.0f / 65536.0f) + 1e-30f;
-    lo = V(lo.x - e, lo.y - e, lo.z - e);
-    hi = V(hi.x + e, hi.y + e, hi.z + e);
-}
-
 __global__ void k_bounds_init(uint32_t* b) {
     int t = threadIdx.x;
     if (t < 12) b[t] = ((t / 3) % 2 == 0) ? 0xffffffffu : 0u;  // lo: min, hi: max

@@ -338,7 +338,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
                     if (__hip_atomic_load(pool, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < r_n)
                         base = atomicAdd(pool, kTailBatch);
                 }
-                base = __shfl(base, 0);
+                // every lane of the wave is active here (no lane leaves the
+                // loop before the whole wave does), so lane 0 is the first
+                // active lane; a scalar copy keeps the range state in SGPRs
+                base = __builtin_amdgcn_readfirstlane(base);
                 if (base < r_n) {
                     b_next = base;
                     b_end = min(r_n, base + kTailBatch);

@@ -74,7 +74,9 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
     uint32_t cn = 0, cs = 0, cq = 0;                  // per-ray counts
     unsigned long long tn = 0, ts = 0, tq = 0;        // per-lane totals
 
-    auto test_prim = [&](uint32_t id) {
+    // te: the entry distance of the primitive's box; a hit before it does not
+    // count (this makes the closest hit independent of the visit order)
+    auto test_prim = [&](uint32_t id, float te) {
         float t;
         bool h;
         if (id >= n_prims) return;
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
             float al, be;
             h = hit_quad(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t, al, be);
         }
-        if (h && (t < best_t || (t == best_t && id < best_id))) {
+        if (h && t >= te && (t < best_t || (t == best_t && id < best_id))) {
             best_t = t;
             best_id = id;
         }
@@ -135,7 +137,10 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
                     if (__hip_atomic_load(pool, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < r_n)
                         base = atomicAdd(pool, batch);
                 }
-                base = __shfl(base, 0);
+                // every lane of the wave is active here (no lane leaves the
+                // loop before the whole wave does), so lane 0 is the first
+                // active lane; a scalar copy keeps the range state in SGPRs
+                base = __builtin_amdgcn_readfirstlane(base);
                 if (base < r_n) {
                     b_next = base;
                     b_end = min(r_n, base + batch);
@@ -184,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
                         if (!slab(r0, r1, o, inv, tmin, best_t, tr)) {
                             finish();
                         } else if (rc & AURORA_NODE_LEAF) {
-                            test_prim(rc & ~AURORA_NODE_LEAF);
+                            test_prim(rc & ~AURORA_NODE_LEAF, tr);
                             finish();
                         } else {
                             cur = rc;
@@ -249,11 +254,11 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
             bool hb = slab(b0, b1, o, inv, tmin, best_t, tb);
             const uint32_t ca = __float_as_uint(a0.w), cb = __float_as_uint(b0.w);
             if (ha && (ca & AURORA_NODE_LEAF)) {
-                test_prim(ca & ~AURORA_NODE_LEAF);
+                test_prim(ca & ~AURORA_NODE_LEAF, ta);
                 ha = false;
             }
             if (hb && (cb & AURORA_NODE_LEAF)) {
-                test_prim(cb & ~AURORA_NODE_LEAF);
+                test_prim(cb & ~AURORA_NODE_LEAF, tb);
                 hb = false;
             }
             if (ha && hb) {
@@ -289,10 +294,18 @@ __global__ __launch_bounds__(kBlock) void k_brute(DevScene sc, const float4* ro,
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const v3 o = xyz(ro[i]), d = xyz(rd[i]);
+    const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float best_t = tmax;
     uint32_t best_id = kNoPrim;
     const uint32_t np = sc.n_spheres + sc.n_quads;
     for (uint32_t id = 0; id < np; ++id) {
+        // the rule of the tree traversals: the ray must enter the
+        // primitive's box, and a hit before the box's entry does not count
+        v3 lo, hi;
+        prim_box(sc, id, lo, hi);
+        float te;
+        if (!slab(make_float4(lo.x, lo.y, lo.z, 0.0f), make_float4(hi.x, hi.y, hi.z, 0.0f), o, inv, tmin, tmax, te))
+            continue;
         float t;
         bool h;
         if (id < sc.n_spheres) {
@@ -302,7 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_brute(DevScene sc, const float4* ro,
             float al, be;
             h = hit_quad(q[0], q[1], q[2], o, d, tmin, tmax, t, al, be);
         }
-        if (h && (t < best_t || (t == best_t && id < best_id))) {
+        if (h && t >= te && (t < best_t || (t == best_t && id < best_id))) {
             best_t = t;
             best_id = id;
         }

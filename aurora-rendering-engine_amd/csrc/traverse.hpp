@@ -6,8 +6,16 @@
 // uses whole lines, so it wins where traversal is memory bound (large
 // incoherent ray queries).  Binary: the 64-byte child pair per step, a
 // cheaper step that wins where traversal is issue bound (render bounces,
-// whose nodes sit in cache).  Ties in t go to the lower primitive id, so
-// the closest hit does not depend on the form or the order.
+// whose nodes sit in cache).
+//
+// Order independence.  A primitive's hit counts only if its t is at least
+// the entry distance of the primitive's own (leaf) box, and ties in t go to
+// the lower primitive id.  Slab arithmetic is monotone in the box bounds
+// (correctly rounded subtract and multiply, min and max), and every
+// ancestor's bounds contain its children's as floats, so an ancestor's entry
+// distance is never above its leaf's, hence never above a counted hit's t:
+// no ancestor of the closest counted hit is culled while the best t is at
+// or above it, and every form and visit order returns the same hit.
 #pragma once
 
 #include "dev_common.hpp"
@@ -33,7 +41,7 @@ __device__ __forceinline__ void wide_cs(uint32_t& ka, uint32_t& ra, uint32_t& kb
 
 // One step of the four-wide traversal at node `cur` (8 float4: per slot
 // (lo, ref), (hi, 0)): slab tests of the four slots against tcur, then the
-// hit leaf slots' primitives in slot order (test_prim), then the hit
+// hit leaf slots' primitives in slot order (test_prim(id, entry t)), then the hit
 // internal slots nearest first by the comparators (0,1) (2,3) (0,2) (1,3)
 // (1,2) (equal distances stay where the network leaves them).  Returns
 // their count; ref[0..count) holds them in that order.
@@ -46,34 +54,37 @@ __device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3
     const float4 l2 = ldg4(W, b + 4), h2 = ldg4(W, b + 5), l3 = ldg4(W, b + 6), h3 = ldg4(W, b + 7);
     const uint32_t f0 = __float_as_uint(l0.w), f1 = __float_as_uint(l1.w);
     const uint32_t f2 = __float_as_uint(l2.w), f3 = __float_as_uint(l3.w);
-    float t0, t1, t2, t3;
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
     bool e0 = f0 != AURORA_BVH4_EMPTY && slab(l0, h0, o, inv, tmin, tcur, t0);
     bool e1 = f1 != AURORA_BVH4_EMPTY && slab(l1, h1, o, inv, tmin, tcur, t1);
     bool e2 = f2 != AURORA_BVH4_EMPTY && slab(l2, h2, o, inv, tmin, tcur, t2);
     bool e3 = f3 != AURORA_BVH4_EMPTY && slab(l3, h3, o, inv, tmin, tcur, t3);
-    // the hit leaf slots, compacted in slot order: the wave runs the j-th
-    // primitive test only when some lane has more than j leaves, instead of
-    // once per slot that any lane hit
+    // the hit leaf slots, compacted in slot order with their entry
+    // distances: the wave runs the j-th primitive test only when some lane
+    // has more than j leaves, instead of once per slot that any lane hit
     uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+    float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f, n3 = 0.0f;
     int np = 0;
-    auto add = [&](bool& e, uint32_t f) {
+    auto add = [&](bool& e, uint32_t f, float tn) {
         if (!(e && (f & AURORA_NODE_LEAF))) return;
         const uint32_t id = f & ~AURORA_NODE_LEAF;
         p0 = np == 0 ? id : p0;
         p1 = np == 1 ? id : p1;
         p2 = np == 2 ? id : p2;
         p3 = np == 3 ? id : p3;
+        n0 = np == 0 ? tn : n0;
+        n1 = np == 1 ? tn : n1;
+        n2 = np == 2 ? tn : n2;
+        n3 = np == 3 ? tn : n3;
         ++np;
         e = false;
     };
-    add(e0, f0);
-    add(e1, f1);
-    add(e2, f2);
-    add(e3, f3);
-    if (np > 0) test_prim(p0);
-    if (np > 1) test_prim(p1);
-    if (np > 2) test_prim(p2);
-    if (np > 3) test_prim(p3);
+    add(e0, f0, t0);
+    add(e1, f1, t1);
+    add(e2, f2, t2);
+    add(e3, f3, t3);
+    // order the hit internal slots first (their entry distances do not
+    // depend on the leaf tests), so the slot words die before the tests
     uint32_t k0 = e0 ? __float_as_uint(t0 + 0.0f) : ~0u, k1 = e1 ? __float_as_uint(t1 + 0.0f) : ~0u;
     uint32_t k2 = e2 ? __float_as_uint(t2 + 0.0f) : ~0u, k3 = e3 ? __float_as_uint(t3 + 0.0f) : ~0u;
     r0 = f0;
@@ -85,7 +96,12 @@ __device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3
     wide_cs(k0, r0, k2, r2);
     wide_cs(k1, r1, k3, r3);
     wide_cs(k1, r1, k2, r2);
-    return (int)e0 + (int)e1 + (int)e2 + (int)e3;
+    const int m = (int)e0 + (int)e1 + (int)e2 + (int)e3;
+    if (np > 0) test_prim(p0, n0);
+    if (np > 1) test_prim(p1, n1);
+    if (np > 2) test_prim(p2, n2);
+    if (np > 3) test_prim(p3, n3);
+    return m;
 }
 
 // Single-ray form of the four-wide traversal, for kernels that trace a ray
@@ -102,7 +118,8 @@ __device__ __forceinline__ void traverse_closest_wide(const DevScene& sc, v3 o, 
     const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float best_t = tmax;
     uint32_t best_id = 0x7fffffffu;
-    auto test_prim = [&](uint32_t id) {
+    // tn: the entry distance of the primitive's box; hits before it do not count
+    auto test_prim = [&](uint32_t id, float tn) {
         if (id >= n_prims) return;
         float t;
         bool h;
@@ -113,7 +130,7 @@ __device__ __forceinline__ void traverse_closest_wide(const DevScene& sc, v3 o, 
             float al, be;
             h = hit_quad(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t, al, be);
         }
-        if (h && (t < best_t || (t == best_t && id < best_id))) {
+        if (h && t >= tn && (t < best_t || (t == best_t && id < best_id))) {
             best_t = t;
             best_id = id;
         }
@@ -166,7 +183,8 @@ __device__ __forceinline__ void traverse_closest_binary(const DevScene& sc, v3 o
     const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float best_t = tmax;
     uint32_t best_id = 0x7fffffffu;
-    auto test_prim = [&](uint32_t id) {
+    // tn: the entry distance of the primitive's box; hits before it do not count
+    auto test_prim = [&](uint32_t id, float tn) {
         if (id >= n_prims) return;
         float t;
         bool h;
@@ -177,7 +195,7 @@ __device__ __forceinline__ void traverse_closest_binary(const DevScene& sc, v3 o
             float al, be;
             h = hit_quad(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t, al, be);
         }
-        if (h && (t < best_t || (t == best_t && id < best_id))) {
+        if (h && t >= tn && (t < best_t || (t == best_t && id < best_id))) {
             best_t = t;
             best_id = id;
         }
@@ -188,7 +206,7 @@ __device__ __forceinline__ void traverse_closest_binary(const DevScene& sc, v3 o
         const uint32_t rc = __float_as_uint(r0.w);
         if (slab(r0, r1, o, inv, tmin, best_t, tr)) {
             if (rc & AURORA_NODE_LEAF) {
-                test_prim(rc & ~AURORA_NODE_LEAF);
+                test_prim(rc & ~AURORA_NODE_LEAF, tr);
             } else {
                 uint32_t cur = rc;
                 int sp = 0;
@@ -201,11 +219,11 @@ __device__ __forceinline__ void traverse_closest_binary(const DevScene& sc, v3 o
                     bool hb = slab(b0, b1, o, inv, tmin, best_t, tb);
                     const uint32_t ca = __float_as_uint(a0.w), cb = __float_as_uint(b0.w);
                     if (ha && (ca & AURORA_NODE_LEAF)) {
-                        test_prim(ca & ~AURORA_NODE_LEAF);
+                        test_prim(ca & ~AURORA_NODE_LEAF, ta);
                         ha = false;
                     }
                     if (hb && (cb & AURORA_NODE_LEAF)) {
-                        test_prim(cb & ~AURORA_NODE_LEAF);
+                        test_prim(cb & ~AURORA_NODE_LEAF, tb);
                         hb = false;
                     }
                     if (ha && hb) {

@@ -41,6 +41,14 @@ void orc_prim_box(orc_scene* s, uint32_t id, float* lo, float* hi);
  * is 4 node records), tree modes only. */
 int orc_trace(orc_scene* s, const float* o, const float* d, uint32_t n, float tmin, float tmax, int brute,
               int32_t* prim, float* t, uint32_t* counts, int threads);
+/* the segments of one path (pixel, sample): per segment origin (3),
+ * direction (3), hit primitive (int32 bits) and hit t, at most cap segments;
+ * returns the segment count, or minus a status code */
+int orc_path_segments(orc_scene* s, const aurora_camera_desc* c, const aurora_render_desc* rd, uint32_t pixel,
+                      uint32_t sample, float* out, uint32_t cap);
+/* the tree later renders walk: 0 the binary LBVH (default), 1 the four-wide
+ * tree; process-wide */
+void orc_set_render_tree(int wide);
 /* the order in which a four-wide step descends into its hit internal slots:
  * t[4] entry distances, hit[4] flags; ord[0..m) receives the hit slots
  * nearest first, the rest the others; returns m */

@@ -44,6 +44,9 @@ def lib():
             "orc_prim_box": (None, [vp, u32, vp, vp]),
             "orc_trace": (C.c_int, [vp, vp, vp, u32, f, f, C.c_int, vp, vp, vp, C.c_int]),
             "orc_wide_order": (C.c_int, [vp, vp, vp]),
+            "orc_set_render_tree": (None, [C.c_int]),
+            "orc_path_segments": (C.c_int, [vp, C.POINTER(abi.CameraDesc), C.POINTER(abi.RenderDesc), u32, u32,
+                                            vp, u32]),
             "orc_hit_sphere": (C.c_int, [vp, vp, vp, f, f, vp]),
             "orc_hit_quad": (C.c_int, [vp, u32, vp, vp, f, f, vp]),
             "orc_texture_value": (None, [vp, u32, f, f, vp, vp]),
@@ -230,8 +233,23 @@ class OracleScene:
         lib().orc_texture_value(self.h, tex, u, v, pp.ctypes.data, out.ctypes.data)
         return out
 
-    def render(self, cam, rs, pixels=None, f64=False, threads=0, aov=None):
-        """Sum of rs.spp samples per pixel (row-major pixel ids or all)."""
+    def path_segments(self, cam, rs, pixel, sample, wide=False, cap=256):
+        """The segments of one path: (origins, directions, prims, ts)."""
+        out = np.zeros((cap, 8), np.float32)
+        c, r = cam.desc(), rs.desc()
+        lib().orc_set_render_tree(1 if wide else 0)
+        try:
+            n = lib().orc_path_segments(self.h, C.byref(c), C.byref(r), pixel, sample, out.ctypes.data, cap)
+        finally:
+            lib().orc_set_render_tree(0)
+        if n < 0:
+            raise RuntimeError(_err())
+        seg = out[:min(n, cap)]
+        return seg[:, 0:3], seg[:, 3:6], seg[:, 6].view(np.int32), seg[:, 7]
+
+    def render(self, cam, rs, pixels=None, f64=False, threads=0, aov=None, wide=False):
+        """Sum of rs.spp samples per pixel (row-major pixel ids or all);
+        wide: walk the four-wide tree instead of the binary one."""
         W, H = cam.image_width, cam.image_height
         n = W * H if pixels is None else len(pixels)
         dt = np.float64 if f64 else np.float32
@@ -243,9 +261,13 @@ class OracleScene:
         segs = C.c_uint64(0)
         fn = lib().orc_render_pixels_f64 if f64 else lib().orc_render_pixels
         c, r = cam.desc(), rs.desc()
-        rc = fn(self.h, C.byref(c), C.byref(r), px.ctypes.data if px is not None else None, n, threads,
-                rgb.ctypes.data, alb.ctypes.data if use_aov else None, nrm.ctypes.data if use_aov else None,
-                C.byref(segs))
+        lib().orc_set_render_tree(1 if wide else 0)
+        try:
+            rc = fn(self.h, C.byref(c), C.byref(r), px.ctypes.data if px is not None else None, n, threads,
+                    rgb.ctypes.data, alb.ctypes.data if use_aov else None, nrm.ctypes.data if use_aov else None,
+                    C.byref(segs))
+        finally:
+            lib().orc_set_render_tree(0)
         if rc == abi.AURORA_ERR_INVALID:
             raise ValueError(_err())
         if rc:

@@ -30,6 +30,14 @@ struct Cam {
     uint32_t W, H;
 };
 
+// the tree the renderer walks: 0 the binary LBVH, 1 the four-wide tree (as
+// aurora_set_render_tree selects on the GPU)
+static std::atomic<int> g_render_wide{0};
+
+// when set (on this thread), trace_path appends per segment: origin (3),
+// direction (3), hit primitive (int32 bits) and hit t
+static thread_local std::vector<float>* g_seg_log = nullptr;
+
 static std::string camera_setup(const aurora_camera_desc& c, Cam& out) {
     if (c.image_width == 0 || c.image_height == 0) return "image width and height must be positive";
     if (!(c.vfov > 0.0f && c.vfov < 180.0f)) return "vfov must be in (0, 180) degrees";
@@ -167,7 +175,17 @@ static V3<R> trace_path(const Scene& s, const Cam& cam, const aurora_render_desc
     const uint32_t ns = uint32_t(s.spheres.size());
     for (uint32_t depth = 0; depth < rd.max_depth; ++depth) {
         segments++;
-        Hit<R> h = closest_bvh<R>(s, org, dir, tmin, R(INFINITY), nullptr);
+        Hit<R> h = g_render_wide.load(std::memory_order_relaxed)
+                       ? closest_bvh4<R>(s, org, dir, tmin, R(INFINITY), nullptr)
+                       : closest_bvh<R>(s, org, dir, tmin, R(INFINITY), nullptr);
+        if (g_seg_log) {
+            float pf;
+            const int32_t pi = h.prim;
+            std::memcpy(&pf, &pi, 4);
+            for (float v : {float(org.x), float(org.y), float(org.z), float(dir.x), float(dir.y), float(dir.z), pf,
+                            float(h.t)})
+                g_seg_log->push_back(v);
+        }
         if (h.prim < 0) {
             L = L + beta * bg;
             if (depth == 0 && alb) { *alb = clamp01(bg); *nrm = mk<R>(0, 0, 0); }
@@ -420,6 +438,24 @@ int orc_trace(orc_scene* sc, const float* o, const float* d, uint32_t n, float t
     });
     return AURORA_OK;
 }
+
+int orc_path_segments(orc_scene* sc, const aurora_camera_desc* c, const aurora_render_desc* rd, uint32_t pixel,
+                      uint32_t sample, float* out, uint32_t cap) {
+    aurora_render_desc one = *rd;
+    one.spp = 1;
+    one.sample_offset = sample;
+    std::vector<float> log;
+    float rgb[3] = {0, 0, 0};
+    g_seg_log = &log;
+    const int rc = render_pixels<float>(sc, c, &one, &pixel, 1, 1, rgb, nullptr, nullptr, nullptr);
+    g_seg_log = nullptr;
+    if (rc) return -rc;
+    const uint32_t n = uint32_t(log.size() / 8);
+    std::memcpy(out, log.data(), std::min(n, cap) * 8 * sizeof(float));
+    return int(n);
+}
+
+void orc_set_render_tree(int wide) { g_render_wide.store(wide ? 1 : 0); }
 
 int orc_wide_order(const float* t, const int32_t* hit, int32_t* ord) {
     bool h[4];

@@ -394,18 +394,26 @@ void build_lbvh(Scene& s) {
 // ----------------------------------------------------- primitive tests ---
 template <class R>
 bool hit_sphere(const Sphere& sp, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
+    // robust roots (Haines et al., Ray Tracing Gems 2019, ch. 7):
+    // h^2 - a*c as a*r^2 - |d x oc|^2, roots c/q and q/a with
+    // q = h + sign(h)*sqrt(disc); the near one (c/q when q > 0, else q/a) if
+    // it lies in (tmin, tmax), else the far one
     V3<R> c = mk<R>(sp.c[0], sp.c[1], sp.c[2]);
     R r = R(sp.r);
     V3<R> oc = c - o;
     R a = dot(d, d);
     R h = dot(d, oc);
-    R cc = dot(oc, oc) - r * r;
-    R disc = h * h - a * cc;
-    if (disc < R(0)) return false;
-    R sq = std::sqrt(disc);
-    R root = (h - sq) / a;
+    R rr = r * r;
+    R cc = dot(oc, oc) - rr;
+    V3<R> x = cross(d, oc);
+    R disc = a * rr - dot(x, x);
+    if (!(disc >= R(0))) return false;
+    R q = h + std::copysign(std::sqrt(disc), h);
+    if (q == R(0)) return false;
+    const bool pos = q > R(0);
+    R root = pos ? cc / q : q / a;
     if (!(root > tmin && root < tmax)) {
-        root = (h + sq) / a;
+        root = pos ? q / a : cc / q;
         if (!(root > tmin && root < tmax)) return false;
     }
     t = root;
@@ -437,8 +445,14 @@ bool hit_quad(const Quad& q, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& al, R& b
 }
 
 template <class R>
-static inline void test_prim(const Scene& s, uint32_t id, V3<R> o, V3<R> d, R tmin, R tmax, Hit<R>& best,
-                             Counts* cnt) {
+static inline void test_prim(const Scene& s, uint32_t id, V3<R> o, V3<R> d, R tmin, R tmax, R tentry,
+                             Hit<R>& best, Counts* cnt) {
+    // tentry: the entry distance of the primitive's box.  A hit before it
+    // does not count; with ties to the lower id, this makes the closest hit
+    // independent of the tree and the visit order (ancestor boxes contain
+    // their children's bounds as floats, and the slab arithmetic is
+    // monotone in them, so no ancestor of a counted hit is culled while the
+    // best t is at or above it)
     uint32_t ns = uint32_t(s.spheres.size());
     R t;
     bool h;
@@ -450,7 +464,7 @@ static inline void test_prim(const Scene& s, uint32_t id, V3<R> o, V3<R> d, R tm
         R a, b;
         h = hit_quad(s.quads[id - ns], o, d, tmin, tmax, t, a, b);
     }
-    if (h && (t < best.t || (t == best.t && int32_t(id) < best.prim))) {
+    if (h && t >= tentry && (t < best.t || (t == best.t && int32_t(id) < best.prim))) {
         best.t = t;
         best.prim = int32_t(id);
     }
@@ -480,7 +494,7 @@ Hit<R> closest_bvh(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt
     if (cnt) cnt->nodes++;
     if (slab(N[0], o, inv, tmin, best.t, tn)) {
         if (N[0].child & AURORA_NODE_LEAF) {
-            test_prim(s, N[0].child & ~AURORA_NODE_LEAF, o, d, tmin, tmax, best, cnt);
+            test_prim(s, N[0].child & ~AURORA_NODE_LEAF, o, d, tmin, tmax, tn, best, cnt);
         } else {
             uint32_t stack[64];
             int sp = 0;
@@ -493,11 +507,11 @@ Hit<R> closest_bvh(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt
                 bool ha = slab(a, o, inv, tmin, best.t, ta);
                 bool hb = slab(b, o, inv, tmin, best.t, tb);
                 if (ha && (a.child & AURORA_NODE_LEAF)) {
-                    test_prim(s, a.child & ~AURORA_NODE_LEAF, o, d, tmin, tmax, best, cnt);
+                    test_prim(s, a.child & ~AURORA_NODE_LEAF, o, d, tmin, tmax, ta, best, cnt);
                     ha = false;
                 }
                 if (hb && (b.child & AURORA_NODE_LEAF)) {
-                    test_prim(s, b.child & ~AURORA_NODE_LEAF, o, d, tmin, tmax, best, cnt);
+                    test_prim(s, b.child & ~AURORA_NODE_LEAF, o, d, tmin, tmax, tb, best, cnt);
                     hb = false;
                 }
                 if (ha && hb) {
@@ -592,7 +606,7 @@ Hit<R> closest_bvh4(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cn
         }
         for (int k = 0; k < 4; ++k)
             if (hit[k] && (w.slot[k].ref & AURORA_NODE_LEAF)) {
-                test_prim(s, w.slot[k].ref & ~AURORA_NODE_LEAF, o, d, tmin, tmax, best, cnt);
+                test_prim(s, w.slot[k].ref & ~AURORA_NODE_LEAF, o, d, tmin, tmax, tn[k], best, cnt);
                 hit[k] = false;
             }
         int ord[4];
@@ -615,7 +629,16 @@ Hit<R> closest_bvh4(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cn
 template <class R>
 Hit<R> closest_brute(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax) {
     Hit<R> best{tmax, 0x7fffffff};
-    for (uint32_t id = 0; id < s.n_prims(); ++id) test_prim<R>(s, id, o, d, tmin, tmax, best, nullptr);
+    // the traversals' rule: the ray must enter the primitive's box (as the
+    // leaf stores it), and a hit before the box's entry does not count
+    const V3<R> inv = mk<R>(R(1) / d.x, R(1) / d.y, R(1) / d.z);
+    for (uint32_t id = 0; id < s.n_prims(); ++id) {
+        aurora_bvh_node nd{};
+        prim_box(s, id, nd.lo, nd.hi);
+        R te;
+        if (!slab(nd, o, inv, tmin, tmax, te)) continue;
+        test_prim<R>(s, id, o, d, tmin, tmax, te, best, nullptr);
+    }
     if (best.prim == 0x7fffffff) { best.prim = AURORA_MISS; best.t = tmax; }
     return best;
 }

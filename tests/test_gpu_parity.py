@@ -153,6 +153,44 @@ def test_closest_hit_2_24_rays_against_brute_force(ctx, name):
     assert np.array_equal(bt[sub].cpu().numpy().view(np.uint32), ot.view(np.uint32))
 
 
+def test_closest_hit_on_silhouettes_independent_of_tree(ctx):
+    """Rays grazing small distant spheres among crossing quads, where a
+    primitive's computed hit can fall outside its box: GPU queries over the
+    four-wide tree, GPU brute force and the oracle agree bit for bit (hits,
+    t and per-ray fetch counts), and a render of the scene is the same with
+    the binary and the four-wide tree and equals the oracle."""
+    say("silhouette rays")
+    arr, org, dirs = scenes.silhouette_scene(1)
+    g, gs, o, os_ = both(ctx, arr)
+    op, ot, oc = o.trace(org, dirs, tmin=1e-3, counts=True)
+    n = len(org)
+    dp = torch.zeros(n, dtype=torch.int32, device="cuda")
+    dt = torch.zeros(n, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros((n, 3), dtype=torch.int32, device="cuda")
+    g.trace_device(dev(org), dev(dirs), dp, dt, tmin=1e-3, counts=cnt)
+    bp = torch.zeros_like(dp)
+    bt = torch.zeros_like(dt)
+    g.trace_device(dev(org), dev(dirs), bp, bt, tmin=1e-3, brute=True)
+    ctx.synchronize()
+    assert (op >= 0).mean() > 0.9
+    assert np.array_equal(dp.cpu().numpy(), op)
+    assert np.array_equal(dt.cpu().numpy().view(np.uint32), ot.view(np.uint32))
+    assert np.array_equal(cnt.cpu().numpy().astype(np.uint32), oc)
+    assert np.array_equal(bp.cpu().numpy(), op)
+    assert np.array_equal(bt.cpu().numpy().view(np.uint32), ot.view(np.uint32))
+    cam = Camera(64, 48, 12.0, (0, 0.2, 4), (0, 0, 0), (0, 1, 0), 0.0, 1.0)
+    rs = RenderSettings(spp=4, max_depth=8, background=(0.7, 0.8, 1.0))
+    orgb, _, _, segs = o.render(cam, rs)
+    for wide in (False, True):
+        ctx.set_render_tree(wide)
+        try:
+            grgb, _, _, st = g.render(cam, rs)
+        finally:
+            ctx.set_render_tree(False)
+        assert st["segments"] == segs
+        assert np.array_equal(grgb.view(np.uint32), orgb.view(np.uint32)), wide
+
+
 def test_closest_hit_axis_rays_and_rays_on_faces(ctx):
     """Directions along the axes (1/d infinite, 0*inf in the slabs) and
     origins on the planes of the Cornell walls and box faces: GPU queries,
