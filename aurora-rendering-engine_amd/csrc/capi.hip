@@ -369,7 +369,11 @@ Status ensure_workspace(aurora_ctx* c, uint32_t cap, int grid) {
     return Status::ok();
 }
 
-constexpr uint32_t kPassPaths = 1u << 23;  // wavefront width per pass
+// Paths per pass when the caller leaves spp_per_pass at 0.  A pass holds
+// 152 bytes of queues per path (two ping-pong queues, hit, radiance and the
+// two AOV planes), so 2^27 paths take 20 GB of the 288 GB of HBM: few, long
+// passes amortise each launch's ramp and drain.
+constexpr uint64_t kPassPaths = 1ull << 27;
 
 // One image of a render call.
 struct Job {
@@ -408,13 +412,35 @@ Status validate_job(aurora_scene* s, const aurora_camera_desc* cam, const aurora
 Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* stats, bool count_fetches,
                    bool graph) {
     aurora_render_stats st{};
-    uint64_t total_wh = 0;
-    for (const Job& j : jobs) total_wh += (uint64_t)j.cam.W * j.cam.H;
-    // samples per pass: explicit, or an equal share of the pass width
+    // samples per pass: explicit (spp_per_pass), or automatic: the fewest
+    // passes P that keep a pass within kPassPaths paths next to the explicit
+    // jobs, each automatic job rendering ceil(spp / P) samples per pass, so
+    // all of them finish in the same pass
+    uint64_t fixed = 0;
+    uint32_t max_spp = 1;
+    for (const Job& j : jobs) {
+        const uint64_t wh = (uint64_t)j.cam.W * j.cam.H;
+        if (j.rd.spp_per_pass) fixed += std::min<uint64_t>(j.rd.spp_per_pass, std::max<uint32_t>(1, j.rd.spp)) * wh;
+        else max_spp = std::max(max_spp, j.rd.spp);
+    }
+    const uint64_t room = kPassPaths > fixed ? kPassPaths - fixed : 0;
+    auto auto_paths = [&](uint64_t P) {
+        uint64_t n = 0;
+        for (const Job& j : jobs)
+            if (!j.rd.spp_per_pass)
+                n += std::max<uint64_t>(1, (j.rd.spp + P - 1) / P) * ((uint64_t)j.cam.W * j.cam.H);
+        return n;
+    };
+    uint64_t P_lo = 1, P_hi = max_spp;  // auto_paths is non-increasing in P; at max_spp every S is 1
+    while (P_lo < P_hi) {
+        const uint64_t mid = (P_lo + P_hi) / 2;
+        if (auto_paths(mid) <= room) P_hi = mid;
+        else P_lo = mid + 1;
+    }
     for (Job& j : jobs) {
         const uint64_t wh = (uint64_t)j.cam.W * j.cam.H;
-        uint64_t S = j.rd.spp_per_pass ? j.rd.spp_per_pass : std::max<uint64_t>(1, kPassPaths / std::max<uint64_t>(1, total_wh));
-        S = std::min<uint64_t>(S, std::max<uint32_t>(1, j.rd.spp));
+        uint64_t S = j.rd.spp_per_pass ? j.rd.spp_per_pass : (j.rd.spp + P_lo - 1) / P_lo;
+        S = std::max<uint64_t>(1, std::min<uint64_t>(S, std::max<uint32_t>(1, j.rd.spp)));
         if (S * wh > 0xffffffffull) return Status{AURORA_ERR_INVALID, "spp_per_pass * pixels exceeds 2^32"};
         j.S = (uint32_t)S;
     }

@@ -178,7 +178,8 @@ typedef struct aurora_render_desc {
     uint32_t seed;           /* Philox key word 0                            */
     float background[3];     /* radiance of rays that leave the scene        */
     uint32_t flags;          /* AURORA_RENDER_*                              */
-    uint32_t spp_per_pass;   /* samples per wavefront pass (0 = automatic)   */
+    uint32_t spp_per_pass;   /* samples per wavefront pass; 0 = automatic:
+                              * the fewest passes of at most 2^27 paths   */
     float tmin;              /* ray offset; <= 0 selects 0.001               */
 } aurora_render_desc;
 
