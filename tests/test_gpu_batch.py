@@ -86,3 +86,24 @@ def test_graph_render_equals_eager(ctx, tail):
         ctx.set_launch()
     assert st["passes"] == 6
     assert torch.equal(a, b)
+
+
+def test_pass_splits_do_not_change_images(ctx):
+    """Samples per pass, explicit (1, 2 or 3) or automatic, mixed within one
+    batch: every image equals its single render in one pass bit for bit
+    (sums are added in sample order whatever the split)."""
+    import dataclasses
+    print("pass splits", flush=True)
+    items = workload(ctx)
+    singles = []
+    for sc, cam, rs in items:
+        rgb = zeros(cam)
+        sc.render_device(cam, dataclasses.replace(rs, spp_per_pass=rs.spp, aov=False), rgb)
+        singles.append(rgb.cpu().numpy())
+    for graph in (False, True):
+        bufs = [zeros(cam) for _, cam, _ in items]
+        its = [(sc, cam, dataclasses.replace(rs, spp_per_pass=(0, 1, 2, 3)[k % 4], aov=False), b, None, None)
+               for k, ((sc, cam, rs), b) in enumerate(zip(items, bufs))]
+        render_batch(ctx, its, graph=graph)
+        for k, (b, s) in enumerate(zip(bufs, singles)):
+            assert np.array_equal(b.cpu().numpy(), s), (graph, k)

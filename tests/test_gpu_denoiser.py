@@ -124,3 +124,24 @@ def test_training_reduces_loss(ctx):
     print("loss", first, "->", last, flush=True)
     # random targets leave an irreducible loss; the fit must still improve
     assert last < 0.9 * first
+
+
+def test_trained_network_gains_over_plain_render(ctx):
+    """What the shipped network gains over the method without it: on
+    held-out Cornell variants (seeds the training never saw) at 256^2, the
+    denoised image of an N-spp render is closer to a 2048-spp target than
+    the plain N-spp mean, by a wide margin (relMSE at least 10x lower at
+    4 and 16 spp, and lower at 256 spp)."""
+    import os
+
+    from aurora_amd import train
+    weights = os.path.join(os.path.dirname(dn.__file__), os.pardir, "weights", "unet_cornell.f32")
+    net = dn.Denoiser(ctx)
+    net.load(weights)
+    print("held-out gain", flush=True)
+    data = train.make_dataset(ctx, [100000 + k for k in range(4)], 256, [4, 16, 256], 2048, progress=False)
+    res = train.evaluate(net, data)
+    print(res, flush=True)
+    for k in (4, 16):
+        assert res[k]["denoised_relmse"] * 10 < res[k]["plain_relmse"], (k, res[k])
+    assert res[256]["denoised_relmse"] < res[256]["plain_relmse"], res[256]
