@@ -40,7 +40,12 @@ struct aurora_ctx {
     int ray_sort = 1;                // ray queries: 0 never sort, 1 large scenes, 2 always
     uint32_t render_wide = 0;        // render bounces over the four-wide tree (else binary)
     Workspace ws;                    // path queues shared by the context's renders
-    SceneRec* host_recs = nullptr;   // pinned staging of the pass's scene table
+    // pinned staging of the passes' scene tables, two in turn: a table is
+    // rewritten only after the copy that read it two passes ago has run
+    // (recs_ev), so passes and calls are enqueued without waiting for the GPU
+    SceneRec* host_recs[2] = {nullptr, nullptr};
+    hipEvent_t recs_ev[2] = {nullptr, nullptr};
+    int recs_turn = 0;
     uint32_t host_recs_cap = 0;
 };
 
@@ -465,9 +470,14 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
     Workspace& w = c->ws;
     AUR_TRY(w.recs.alloc(jobs.size() * sizeof(SceneRec)));
     if (c->host_recs_cap < jobs.size()) {
-        if (c->host_recs) (void)hipHostFree(c->host_recs);
-        c->host_recs = nullptr;
-        AUR_HIP(hipHostMalloc((void**)&c->host_recs, jobs.size() * sizeof(SceneRec), hipHostMallocDefault));
+        for (int k = 0; k < 2; ++k) {
+            AUR_HIP(hipEventSynchronize(c->recs_ev[k]));  // no copy still reads the old table
+            if (c->host_recs[k]) (void)hipHostFree(c->host_recs[k]);
+            c->host_recs[k] = nullptr;
+        }
+        c->host_recs_cap = 0;
+        for (int k = 0; k < 2; ++k)
+            AUR_HIP(hipHostMalloc((void**)&c->host_recs[k], jobs.size() * sizeof(SceneRec), hipHostMallocDefault));
         c->host_recs_cap = (uint32_t)jobs.size();
     }
     if (count_fetches) {
@@ -501,13 +511,23 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
     uint64_t total_segments = 0;
 
     // build the scene table of the next pass; returns its shape
-    auto make_pass = [&](Pass& pa, bool& same_shape, uint32_t& prev_sig) -> bool {
+    // the table staged for the next pass, and its copy to the device
+    SceneRec* hrecs = nullptr;
+    auto stage_recs = [&]() -> Status {
+        AUR_HIP(hipMemcpyAsync(w.recs.p, hrecs, jobs.size() * sizeof(SceneRec), hipMemcpyHostToDevice, c->stream));
+        AUR_HIP(hipEventRecord(c->recs_ev[c->recs_turn], c->stream));
+        c->recs_turn ^= 1;
+        return Status::ok();
+    };
+    auto make_pass = [&](Pass& pa, bool& same_shape, uint32_t& prev_sig, bool& any) -> Status {
         uint32_t path_off = 0, pix_off = 0, sig = 0;
-        bool any = false;
+        any = false;
+        AUR_HIP(hipEventSynchronize(c->recs_ev[c->recs_turn]));
+        hrecs = c->host_recs[c->recs_turn];
         for (size_t k = 0; k < jobs.size(); ++k) {
             Job& j = jobs[k];
             const uint32_t sp = j.done < j.rd.spp ? std::min(j.S, j.rd.spp - j.done) : 0u;
-            SceneRec& r = c->host_recs[k];
+            SceneRec& r = hrecs[k];
             r.sc = j.scene->view;
             r.cam = j.cam;
             r.seed = j.rd.seed;
@@ -535,7 +555,7 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         pa.n_pixels = pix_off;
         same_shape = sig == prev_sig;
         prev_sig = sig;
-        return any;
+        return Status::ok();
     };
     auto finish_pass = [&](const Pass& pa) {
         for (Job& j : jobs) j.done += j.done < j.rd.spp ? std::min(j.S, j.rd.spp - j.done) : 0u;
@@ -543,13 +563,9 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         st.paths += pa.n_paths;
         st.max_live = std::max(st.max_live, pa.n_paths);
     };
-    auto read_segments = [&]() -> Status {
-        unsigned long long v = 0;
-        AUR_HIP(hipMemcpyAsync(&v, w.counters.as<uint32_t>() + kTailSegs, 8, hipMemcpyDeviceToHost, c->stream));
-        AUR_HIP(hipStreamSynchronize(c->stream));
-        total_segments += v;
-        return Status::ok();
-    };
+    // the kernels add the segments they trace to one device counter over
+    // the whole call; it is read once, at the end, and only for statistics
+    AUR_HIP(hipMemsetAsync(w.counters.as<uint32_t>() + kTailSegs, 0, 8, c->stream));
     auto extend_args = [&](int cur) {
         ExtendArgs a{};
         a.ray_o = w.q_o[cur].as<float4>();
@@ -575,17 +591,17 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
     for (;;) {
         Pass pa{};
         bool same = false;
-        if (!make_pass(pa, same, prev_sig)) break;
+        bool any = false;
+        AUR_TRY(make_pass(pa, same, prev_sig, any));
+        if (!any) break;
         if (graph && same && gexec) {
             // same pass shape as the captured one: replay (the graph's last
             // kernel advanced the device table's sample bases)
             AUR_HIP(hipGraphLaunch(gexec, c->stream));
-            AUR_TRY(read_segments());
             finish_pass(pa);
             continue;
         }
-        AUR_HIP(hipMemcpyAsync(w.recs.p, c->host_recs, jobs.size() * sizeof(SceneRec), hipMemcpyHostToDevice,
-                               c->stream));
+        AUR_TRY(stage_recs());
         if (graph) {
             // capture: fixed launch shapes, no host round trips inside
             if (gexec) {
@@ -616,7 +632,6 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
             AUR_HIP(ce);
             AUR_HIP(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0));
             AUR_HIP(hipGraphLaunch(gexec, c->stream));
-            AUR_TRY(read_segments());
             finish_pass(pa);
             continue;
         }
@@ -670,11 +685,16 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
             AUR_HIP(hipEventRecord(cnt_ev[depth], c->stream));
         }
         AUR_TRY(launch_accumulate(pa, w, c->stream));
-        AUR_TRY(read_segments());
         finish_pass(pa);
     }
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (g) (void)hipGraphDestroy(g);
+    if (stats) {
+        unsigned long long v = 0;
+        AUR_HIP(hipMemcpyAsync(&v, w.counters.as<uint32_t>() + kTailSegs, 8, hipMemcpyDeviceToHost, c->stream));
+        AUR_HIP(hipStreamSynchronize(c->stream));
+        total_segments = v;
+    }
     st.segments = total_segments;
     if (timed) {
         AUR_HIP(hipEventRecord(t_end, c->stream));
@@ -745,7 +765,12 @@ int aurora_create(int device, aurora_ctx** out) {
     // a blocking stream: ordered with work on the legacy default stream
     // (where callers such as PyTorch place their copies and fills)
     if (!s.bad()) s = hip_status(hipStreamCreate(&c->stream), "hipStreamCreate");
+    for (int k = 0; k < 2 && !s.bad(); ++k)
+        s = hip_status(hipEventCreateWithFlags(&c->recs_ev[k], hipEventDisableTiming), "hipEventCreate");
     if (s.bad()) {
+        for (int k = 0; k < 2; ++k)
+            if (c->recs_ev[k]) (void)hipEventDestroy(c->recs_ev[k]);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(nullptr, s);
     }
@@ -764,7 +789,10 @@ void aurora_destroy(aurora_ctx* c) {
                       &w.aov_a, &w.aov_n, &w.counters, &w.spill, &w.recs};
     for (DevBuf* b : bufs) b->release();
     if (w.host_counts) (void)hipHostFree(w.host_counts);
-    if (c->host_recs) (void)hipHostFree(c->host_recs);
+    for (int k = 0; k < 2; ++k) {
+        if (c->host_recs[k]) (void)hipHostFree(c->host_recs[k]);
+        if (c->recs_ev[k]) (void)hipEventDestroy(c->recs_ev[k]);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

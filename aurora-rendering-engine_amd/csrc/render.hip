@@ -48,8 +48,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Pass pa, float4* qo, float4* 
     const uint32_t n = pa.n_paths;
     if (p == 0) {
         counters[0] = n;   // queue 0 count
-        counters[1] = 0u;  // queue 1 count
-        *reinterpret_cast<unsigned long long*>(counters + kTailSegs) = 0ull;
+        counters[1] = 0u;  // queue 1 count (the segment counter spans the call)
     }
     if (p < kPools) counters[kPoolBase + kPoolStride * p] = 0u;  // extend work pools
     if (p >= n) return;

