@@ -400,16 +400,15 @@ def main():
         step(i)
     torch.cuda.synchronize()
     ctx.synchronize()
-    # timed steps are enqueued without statistics (a call that returns them
-    # waits for its pass); the same steps are then rerun untimed with
-    # statistics, which are the same for the same samples
+    # every timed step returns its statistics, with the HIP-event time of
+    # each kernel launch recorded on the library stream inside the timed region
+    steps = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step(args.warmup + i, stats=False)
+        steps.append(step(args.warmup + i))
     ctx.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    steps = [step(args.warmup + i) for i in range(args.steps)]
 
     paths = sum(s["paths"] for s in steps)
     segments = sum(s["segments"] for s in steps)
