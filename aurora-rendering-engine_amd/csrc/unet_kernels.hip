@@ -157,6 +157,126 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn in, const u16* __restric
     }
 }
 
+// Same convolution with the input tile staged in LDS: a block of TR waves
+// computes TR rows x 64 pixels x (16*NT) channels.  It first copies the
+// (TR+2) x 66-pixel input tile (the halo included, zeros outside the image,
+// the upsampled half of a concat resolved) into LDS with coalesced 16-byte
+// loads, once, instead of every wave gathering each of its 9 taps from
+// L1/L2.  A pixel's channels sit at a stride of 2*cin + 16 bytes: 16
+// consecutive pixels then start in distinct LDS banks, so an A fragment
+// (16 pixels x 8 channels per k group) reads without bank conflicts.
+__host__ __device__ constexpr int lds_pix_stride(int cin) { return 2 * cin + 16; }
+__host__ __device__ constexpr int lds_tile_bytes(int tr, int cin) { return (tr + 2) * 66 * lds_pix_stride(cin); }
+
+template <int NT, bool RELU, bool OUT_F32, bool ACCUM>
+__global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __restrict__ wk, int kpad,
+                                                     const float* __restrict__ bias, void* out, int N, int H, int W,
+                                                     int out_stride, int co_base) {
+    extern __shared__ __align__(16) unsigned char tile[];
+    const int TR = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int segs = (W + 63) / 64, bands = (H + TR - 1) / TR;
+    const int x0 = (blockIdx.x % segs) * 64;
+    const int band = (blockIdx.x / segs) % bands;
+    const int n = blockIdx.x / (segs * bands);
+    const int y0 = band * TR;
+    const int cin = in.ca + in.cb;
+    const int ps = lds_pix_stride(cin);
+    const int groups = cin >> 3;  // 16-byte channel groups per pixel
+    // stage the tile: rows y0-1 .. y0+TR, pixels x0-1 .. x0+64
+    const int chunks = (TR + 2) * 66 * groups;
+    for (int i = threadIdx.x; i < chunks; i += blockDim.x) {
+        const int g = i % groups, px = (i / groups) % 66, r = i / (groups * 66);
+        const int y = y0 - 1 + r, x = x0 - 1 + px;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (y >= 0 && y < H && x >= 0 && x < W) {
+            const int c = 8 * g;
+            if (c < in.ca) {
+                v = *reinterpret_cast<const uint4*>(in.a + (((size_t)n * H + y) * W + x) * in.ca + c);
+            } else {
+                const int h2 = H >> 1, w2 = W >> 1;
+                v = *reinterpret_cast<const uint4*>(in.b + (((size_t)n * h2 + (y >> 1)) * w2 + (x >> 1)) * in.cb +
+                                                    (c - in.ca));
+            }
+        }
+        *reinterpret_cast<uint4*>(tile + (r * 66 + px) * ps + 16 * g) = v;
+    }
+    __syncthreads();
+    const int y = y0 + wave;
+    if (y >= H) return;  // no barrier follows
+    const int K = 9 * cin;
+    const int m = lane & 15;   // pixel of the A fragment within its row tile
+    const int kg = lane >> 4;  // k group of 8
+    floatx4 acc[4][NT];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[mt][t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x8 a[4], b[NT];
+    auto load = [&](int k0, bf16x8* fa, bf16x8* fb) {
+        const int k = k0 + 8 * kg;
+        if (k < K) {
+            const int tap = k / cin, c = k - tap * cin;
+            const int row = wave + tap / 3;         // tile row of input row y + dy
+            const int px = m + tap % 3;             // tile pixel of input pixel x0 + m + dx
+            const unsigned char* src = tile + (row * 66 + px) * ps + 2 * c;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                Pack8 v;
+                v.u = *reinterpret_cast<const uint4*>(src + 16 * mt * ps);
+                fa[mt] = v.v;
+            }
+        } else {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) fa[mt] = zero8();
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            Pack8 w;
+            w.u = *reinterpret_cast<const uint4*>(wk + (size_t)(t * 16 + m) * kpad + k);
+            fb[t] = w.v;
+        }
+    };
+    load(0, a, b);
+    for (int k0 = 0; k0 < kpad; k0 += 32) {
+        bf16x8 na[4], nb[NT];
+        const bool more = k0 + 32 < kpad;
+        if (more) load(k0 + 32, na, nb);
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[t], acc[mt][t], 0, 0, 0);
+        if (more) {
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) a[mt] = na[mt];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) b[t] = nb[t];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + m;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = x0 + 16 * mt + 4 * kg + r;
+                if (x >= W) continue;
+                float v = acc[mt][t][r] + bv;
+                if (RELU) v = fmaxf(v, 0.0f);
+                const size_t o = (((size_t)n * H + y) * W + x) * out_stride + co_base + co;
+                if (OUT_F32) {
+                    float* of = static_cast<float*>(out);
+                    of[o] = ACCUM ? of[o] + v : v;
+                } else {
+                    static_cast<u16*>(out)[o] = f2bf(v);
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------- weight gradient ---
 // dW[co][tap][ci] = sum_p g[p][co] * X[p + tap][ci] over the rows of this
 // block: MFMA with M = 16 output channels, N = 16 input channels, K = 32
@@ -453,14 +573,24 @@ Status conv3x3_forward(const ConvIn& in, const uint16_t* wk_all, int kpad, const
                        bool out_f32, bool accum, bool relu, int N, int H, int W, int cout, hipStream_t s) {
     if ((in.ca % 16) || (in.cb % 16) || (cout % 16) || cout < 16 || (in.ca + in.cb) < 16)
         return Status{AURORA_ERR_INVALID, "conv3x3: channel counts must be multiples of 16"};
+    const int cin = in.ca + in.cb;
+    // the LDS-tiled kernel where a 4-row tile fits 64 KB (up to 64 input
+    // channels); the wider decoder inputs gather from L1/L2 instead (2-row
+    // tiles measured slower there: twice the halo rows and half the waves)
+    const int tr = lds_tile_bytes(4, cin) <= 65536 ? 4 : 0;
     const unsigned grid = (unsigned)(((size_t)N * H * ((W + 63) / 64) + 3) / 4);  // 4 row segments per block
+    const unsigned grid_lds = tr ? (unsigned)((size_t)N * ((H + tr - 1) / tr) * ((W + 63) / 64)) : 0u;
+    const size_t lds = tr ? (size_t)lds_tile_bytes(tr, cin) : 0;
     // output channels in groups of at most 64 (4 MFMA tiles per wave)
     for (int co_base = 0; co_base < cout; co_base += 64) {
     const int nt = std::min(64, cout - co_base) / 16;
     const uint16_t* wk = wk_all + (size_t)co_base * kpad;
     const float* bias = bias_all ? bias_all + co_base : nullptr;
-#define AUR_CONV(NTv, R, F, A) \
-    k_conv3x3<NTv, R, F, A><<<grid, 256, 0, s>>>(in, wk, kpad, bias, out, N, H, W, cout, co_base)
+#define AUR_CONV(NTv, R, F, A)                                                                                  \
+    if (tr)                                                                                                    \
+        k_conv3x3_lds<NTv, R, F, A><<<grid_lds, 64 * tr, lds, s>>>(in, wk, kpad, bias, out, N, H, W, cout, co_base); \
+    else                                                                                                       \
+        k_conv3x3<NTv, R, F, A><<<grid, 256, 0, s>>>(in, wk, kpad, bias, out, N, H, W, cout, co_base)
 #define AUR_CONV_NT(R, F, A)          \
     switch (nt) {                     \
         case 1: AUR_CONV(1, R, F, A); break; \
