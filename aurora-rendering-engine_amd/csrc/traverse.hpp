@@ -40,11 +40,12 @@ __device__ __forceinline__ void wide_cs(uint32_t& ka, uint32_t& ra, uint32_t& kb
 }
 
 // One step of the four-wide traversal at node `cur` (8 float4: per slot
-// (lo, ref), (hi, 0)): slab tests of the four slots against tcur, then the
-// hit leaf slots' primitives in slot order (test_prim(id, entry t)), then the hit
-// internal slots nearest first by the comparators (0,1) (2,3) (0,2) (1,3)
-// (1,2) (equal distances stay where the network leaves them).  Returns
-// their count; ref[0..count) holds them in that order.
+// (lo, ref), (hi, 0)): slab tests of the four slots against tcur; the hit
+// internal slots ordered nearest first by the comparators (0,1) (2,3) (0,2)
+// (1,3) (1,2) (equal distances stay where the network leaves them); the hit
+// leaf slots' primitives tested in slot order (test_prim(id, entry t)).
+// Returns the count of hit internal slots; ref[0..count) holds them in that
+// order.
 template <class TestPrim>
 __device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3 inv, float tmin, float tcur,
                                          TestPrim&& test_prim, uint32_t& r0, uint32_t& r1, uint32_t& r2,
