@@ -178,7 +178,7 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=4096):
         dn.load(WEIGHTS)
         dcurve = []
         best = None
-        for spp in (32, 64, 96, 128, 160, 192, 224, 256, 384, 512, 1024, 2048, 4096):
+        for spp in (16, 24, 32, 40, 48, 56, 64, 96, 128, 160, 192, 224, 256, 384, 512, 1024, 2048, 4096):
             rgb, alb, nrm = buf(), buf(), buf()
             tr = render_into(rgb, spp, 0, alb, nrm)
             t0 = time.perf_counter()
