@@ -314,6 +314,47 @@ def test_render_aov_and_progressive(ctx):
     assert np.array_equal(a, orgb)
 
 
+def test_render_defocus_and_odd_sizes(ctx):
+    """A thin-lens camera (defocus disk sampled by Philox draws with
+    rejection, as the oracle does), a 1x1 image, a 7x3 image, a single
+    bounce and an explicit ray offset: radiance bit-exact against the oracle
+    with both render trees."""
+    say("render defocus and odd sizes")
+    arr = scenes.random_spheres(0)
+    g, gs, o, os_ = both(ctx, arr)
+    cases = [
+        (Camera(40, 24, 20.0, (13, 2, 3), (0, 0, 0), (0, 1, 0), 0.6, 10.0),
+         RenderSettings(spp=4, max_depth=12, background=(0.7, 0.8, 1.0))),
+        (Camera(1, 1, 20.0, (13, 2, 3), (0, 0, 0), (0, 1, 0), 0.0, 10.0),
+         RenderSettings(spp=9, max_depth=50, background=(0.7, 0.8, 1.0))),
+        (Camera(7, 3, 30.0, (13, 2, 3), (0, 0, 0), (0, 1, 0), 0.0, 10.0),
+         RenderSettings(spp=3, max_depth=1, background=(0.7, 0.8, 1.0), tmin=1e-3)),
+    ]
+    for cam, rs in cases:
+        orgb, _, _, segs = o.render(cam, rs)
+        for wide in (False, True):
+            ctx.set_render_tree(wide)
+            try:
+                grgb, _, _, st = g.render(cam, rs)
+            finally:
+                ctx.set_render_tree(False)
+            assert st["segments"] == segs, (cam.image_width, wide)
+            assert np.array_equal(grgb.view(np.uint32), orgb.view(np.uint32)), (cam.image_width, wide)
+
+
+def test_camera_errors_are_reported(ctx):
+    say("camera errors")
+    g = ctx.upload(scenes.cornell_box())
+    g.build_bvh()
+    rs = RenderSettings(spp=1)
+    for cam, msg in [(Camera(8, 8, 40.0, (1, 2, 3), (1, 2, 3), (0, 1, 0), 0.0, 1.0), "coincide"),
+                     (Camera(8, 8, 40.0, (0, 0, 0), (0, 1, 0), (0, 1, 0), 0.0, 1.0), "parallel"),
+                     (Camera(8, 8, 180.0, (0, 0, 0), (0, 0, 1), (0, 1, 0), 0.0, 1.0), "vfov"),
+                     (Camera(8, 0, 40.0, (0, 0, 0), (0, 0, 1), (0, 1, 0), 0.0, 1.0), "positive")]:
+        with pytest.raises(ValueError, match=msg):
+            g.render(cam, rs)
+
+
 def test_render_closed_form_light(ctx):
     say("render closed form")
     b = scenes.Builder()
