@@ -6,8 +6,8 @@ independent halves (disjoint sample ranges, so independent Philox streams)
 gives E[(a - b)^2] = 4 Var(mean) for a plain mean, and
 relMSE(D(a), D(b)) / 4 is the same estimate for the denoised output (its
 variance part; bias is not seen, which is why the plain render stays the
-judge: `render_denoised` hands the image over to plain rendering whenever the
-estimate exceeds the budget).
+reference: `render_denoised` hands the image over to plain rendering whenever
+the estimate exceeds the budget).
 """
 import time
 
