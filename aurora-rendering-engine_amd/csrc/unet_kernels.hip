@@ -157,49 +157,58 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn in, const u16* __restric
     }
 }
 
-// Same convolution with the input tile staged in LDS: a block of TR waves
-// computes TR rows x 64 pixels x (16*NT) channels.  It first copies the
-// (TR+2) x 66-pixel input tile (the halo included, zeros outside the image,
-// the upsampled half of a concat resolved) into LDS with coalesced 16-byte
-// loads, once, instead of every wave gathering each of its 9 taps from
-// L1/L2.  A pixel's channels sit at a stride of 2*cin + 16 bytes: 16
-// consecutive pixels then start in distinct LDS banks, so an A fragment
-// (16 pixels x 8 channels per k group) reads without bank conflicts.
-__host__ __device__ constexpr int lds_pix_stride(int cin) { return 2 * cin + 16; }
-__host__ __device__ constexpr int lds_tile_bytes(int tr, int cin) { return (tr + 2) * 66 * lds_pix_stride(cin); }
+// Same convolution with the input tile staged in LDS: a block of 4 waves
+// computes 4 rows x 64 pixels x (16*NT) channels.  It first copies its input
+// tile into LDS with coalesced 16-byte loads, once, instead of every wave
+// gathering each of its 9 taps from L1/L2: the full-resolution part (ca
+// channels) as 6 x 66 pixels, the halo included, and the half-resolution
+// part of a concat (cb channels, read through nearest upsampling) at its own
+// resolution, 4 x 34 pixels; zeros outside the image.  A pixel's channels sit
+// at a stride of 2*C + 16 bytes: 16 consecutive pixels then start in
+// distinct LDS banks, so an A fragment (16 pixels x 8 channels per k group)
+// reads without bank conflicts.
+__host__ __device__ constexpr int lds_pix_stride(int c) { return 2 * c + 16; }
+__host__ __device__ constexpr int lds_tile_bytes(int ca, int cb) {
+    return 6 * 66 * lds_pix_stride(ca) + (cb ? 4 * 34 * lds_pix_stride(cb) : 0);
+}
 
-template <int NT, bool RELU, bool OUT_F32, bool ACCUM>
+// HB: the input has a half-resolution part (cb > 0)
+template <int NT, bool RELU, bool OUT_F32, bool ACCUM, bool HB>
 __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __restrict__ wk, int kpad,
                                                      const float* __restrict__ bias, void* out, int N, int H, int W,
                                                      int out_stride, int co_base) {
     extern __shared__ __align__(16) unsigned char tile[];
-    const int TR = blockDim.x >> 6;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int segs = (W + 63) / 64, bands = (H + TR - 1) / TR;
+    const int segs = (W + 63) / 64, bands = (H + 3) / 4;
     const int x0 = (blockIdx.x % segs) * 64;
     const int band = (blockIdx.x / segs) % bands;
     const int n = blockIdx.x / (segs * bands);
-    const int y0 = band * TR;
-    const int cin = in.ca + in.cb;
-    const int ps = lds_pix_stride(cin);
-    const int groups = cin >> 3;  // 16-byte channel groups per pixel
-    // stage the tile: rows y0-1 .. y0+TR, pixels x0-1 .. x0+64
-    const int chunks = (TR + 2) * 66 * groups;
-    for (int i = threadIdx.x; i < chunks; i += blockDim.x) {
-        const int g = i % groups, px = (i / groups) % 66, r = i / (groups * 66);
+    const int y0 = band * 4;
+    const int ca = in.ca, cb = HB ? in.cb : 0, cin = ca + cb;
+    const int psa = lds_pix_stride(ca), psb = lds_pix_stride(cb);
+    unsigned char* const tb = tile + 6 * 66 * psa;  // the half-resolution part
+    // full-resolution part: rows y0-1 .. y0+4, pixels x0-1 .. x0+64
+    const int ga = ca >> 3;  // 16-byte channel groups per pixel
+    for (int i = threadIdx.x; i < 6 * 66 * ga; i += blockDim.x) {
+        const int g = i % ga, px = (i / ga) % 66, r = i / (ga * 66);
         const int y = y0 - 1 + r, x = x0 - 1 + px;
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (y >= 0 && y < H && x >= 0 && x < W) {
-            const int c = 8 * g;
-            if (c < in.ca) {
-                v = *reinterpret_cast<const uint4*>(in.a + (((size_t)n * H + y) * W + x) * in.ca + c);
-            } else {
-                const int h2 = H >> 1, w2 = W >> 1;
-                v = *reinterpret_cast<const uint4*>(in.b + (((size_t)n * h2 + (y >> 1)) * w2 + (x >> 1)) * in.cb +
-                                                    (c - in.ca));
-            }
-        }
-        *reinterpret_cast<uint4*>(tile + (r * 66 + px) * ps + 16 * g) = v;
+        if (y >= 0 && y < H && x >= 0 && x < W)
+            v = *reinterpret_cast<const uint4*>(in.a + (((size_t)n * H + y) * W + x) * ca + 8 * g);
+        *reinterpret_cast<uint4*>(tile + (r * 66 + px) * psa + 16 * g) = v;
+    }
+    // half-resolution part: the half-rows and half-pixels the full-resolution
+    // rows y0-1 .. y0+4 and pixels x0-1 .. x0+64 fall in (arithmetic shifts:
+    // -1 maps to -1, outside)
+    const int hy0 = (y0 - 1) >> 1, hx0 = (x0 - 1) >> 1, h2 = H >> 1, w2 = W >> 1;
+    const int gb = cb >> 3;
+    for (int i = threadIdx.x; HB && i < 4 * 34 * gb; i += blockDim.x) {
+        const int g = i % gb, px = (i / gb) % 34, r = i / (gb * 34);
+        const int y = hy0 + r, x = hx0 + px;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (y >= 0 && y < h2 && x >= 0 && x < w2)
+            v = *reinterpret_cast<const uint4*>(in.b + (((size_t)n * h2 + y) * w2 + x) * cb + 8 * g);
+        *reinterpret_cast<uint4*>(tb + (r * 34 + px) * psb + 16 * g) = v;
     }
     __syncthreads();
     const int y = y0 + wave;
@@ -217,14 +226,26 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
         const int k = k0 + 8 * kg;
         if (k < K) {
             const int tap = k / cin, c = k - tap * cin;
-            const int row = wave + tap / 3;         // tile row of input row y + dy
-            const int px = m + tap % 3;             // tile pixel of input pixel x0 + m + dx
-            const unsigned char* src = tile + (row * 66 + px) * ps + 2 * c;
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            if (!HB || c < ca) {
+                // tile row of input row y + dy, tile pixel of x0 + m + dx
+                const unsigned char* src = tile + ((wave + dy + 1) * 66 + m + dx + 1) * psa + 2 * c;
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt) {
-                Pack8 v;
-                v.u = *reinterpret_cast<const uint4*>(src + 16 * mt * ps);
-                fa[mt] = v.v;
+                for (int mt = 0; mt < 4; ++mt) {
+                    Pack8 v;
+                    v.u = *reinterpret_cast<const uint4*>(src + 16 * mt * psa);
+                    fa[mt] = v.v;
+                }
+            } else {
+                // input pixel (y + dy, x) read at half resolution
+                const int hr = ((y + dy) >> 1) - hy0;
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) {
+                    const int hp = ((x0 + 16 * mt + m + dx) >> 1) - hx0;
+                    Pack8 v;
+                    v.u = *reinterpret_cast<const uint4*>(tb + (hr * 34 + hp) * psb + 2 * (c - ca));
+                    fa[mt] = v.v;
+                }
             }
         } else {
 #pragma unroll
@@ -574,21 +595,24 @@ Status conv3x3_forward(const ConvIn& in, const uint16_t* wk_all, int kpad, const
     if ((in.ca % 16) || (in.cb % 16) || (cout % 16) || cout < 16 || (in.ca + in.cb) < 16)
         return Status{AURORA_ERR_INVALID, "conv3x3: channel counts must be multiples of 16"};
     const int cin = in.ca + in.cb;
-    // the LDS-tiled kernel where a 4-row tile fits 64 KB (up to 64 input
-    // channels); the wider decoder inputs gather from L1/L2 instead (2-row
-    // tiles measured slower there: twice the halo rows and half the waves)
-    const int tr = lds_tile_bytes(4, cin) <= 65536 ? 4 : 0;
+    // the LDS-tiled kernel where its tiles fit 64 KB (every layer but the
+    // level-2 decoder input, 64 + 64 channels, which gathers from L1/L2)
+    const int tr = lds_tile_bytes(in.ca, in.cb) <= 65536 ? 4 : 0;
     const unsigned grid = (unsigned)(((size_t)N * H * ((W + 63) / 64) + 3) / 4);  // 4 row segments per block
-    const unsigned grid_lds = tr ? (unsigned)((size_t)N * ((H + tr - 1) / tr) * ((W + 63) / 64)) : 0u;
-    const size_t lds = tr ? (size_t)lds_tile_bytes(tr, cin) : 0;
+    const unsigned grid_lds = tr ? (unsigned)((size_t)N * ((H + 3) / 4) * ((W + 63) / 64)) : 0u;
+    const size_t lds = tr ? (size_t)lds_tile_bytes(in.ca, in.cb) : 0;
     // output channels in groups of at most 64 (4 MFMA tiles per wave)
     for (int co_base = 0; co_base < cout; co_base += 64) {
     const int nt = std::min(64, cout - co_base) / 16;
     const uint16_t* wk = wk_all + (size_t)co_base * kpad;
     const float* bias = bias_all ? bias_all + co_base : nullptr;
 #define AUR_CONV(NTv, R, F, A)                                                                                  \
-    if (tr)                                                                                                    \
-        k_conv3x3_lds<NTv, R, F, A><<<grid_lds, 64 * tr, lds, s>>>(in, wk, kpad, bias, out, N, H, W, cout, co_base); \
+    if (tr && in.cb)                                                                                           \
+        k_conv3x3_lds<NTv, R, F, A, true><<<grid_lds, 64 * tr, lds, s>>>(in, wk, kpad, bias, out, N, H, W, cout,   \
+                                                                        co_base);                              \
+    else if (tr)                                                                                               \
+        k_conv3x3_lds<NTv, R, F, A, false><<<grid_lds, 64 * tr, lds, s>>>(in, wk, kpad, bias, out, N, H, W, cout,  \
+                                                                         co_base);                             \
     else                                                                                                       \
         k_conv3x3<NTv, R, F, A><<<grid, 256, 0, s>>>(in, wk, kpad, bias, out, N, H, W, cout, co_base)
 #define AUR_CONV_NT(R, F, A)          \
