@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn in, const u16* __restric
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[t], acc[mt][t], 0, 0, 0);
+            for (int t = 0; t < NT; ++t) acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[t], a[mt], acc[mt][t], 0, 0, 0);
         if (more) {
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) a[mt] = na[mt];
@@ -133,25 +133,39 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn in, const u16* __restric
             for (int t = 0; t < NT; ++t) b[t] = nb[t];
         }
     }
+    // the MFMAs computed each tile transposed (weights as the A operand),
+    // so this lane holds channels co .. co+3 of pixel x: one 8-byte (BF16)
+    // or 16-byte (FP32) store each; out_stride, co_base and co are multiples
+    // of 4, so the stores are aligned
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + m;
-        const float bv = bias ? bias[co] : 0.0f;
+        const int co = t * 16 + 4 * kg;
+        float bv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bv[r] = bias ? bias[co + r] : 0.0f;
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
+            const int x = x0 + 16 * mt + m;
+            if (x >= W) continue;
+            float v[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int x = x0 + 16 * mt + 4 * kg + r;
-                if (x >= W) continue;
-                float v = acc[mt][t][r] + bv;
-                if (RELU) v = fmaxf(v, 0.0f);
-                const size_t o = (((size_t)n * H + y) * W + x) * out_stride + co_base + co;
-                if (OUT_F32) {
-                    float* of = static_cast<float*>(out);
-                    of[o] = ACCUM ? of[o] + v : v;
-                } else {
-                    static_cast<u16*>(out)[o] = f2bf(v);
+                v[r] = acc[mt][t][r] + bv[r];
+                if (RELU) v[r] = fmaxf(v[r], 0.0f);
+            }
+            const size_t o = (((size_t)n * H + y) * W + x) * out_stride + co_base + co;
+            if (OUT_F32) {
+                float4* of = reinterpret_cast<float4*>(static_cast<float*>(out) + o);
+                float4 w4 = make_float4(v[0], v[1], v[2], v[3]);
+                if (ACCUM) {
+                    const float4 c4 = *of;
+                    w4 = make_float4(c4.x + v[0], c4.y + v[1], c4.z + v[2], c4.w + v[3]);
                 }
+                *of = w4;
+            } else {
+                const uint2 h = make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                                           (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+                *reinterpret_cast<uint2*>(static_cast<u16*>(out) + o) = h;
             }
         }
     }
@@ -266,7 +280,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[t], acc[mt][t], 0, 0, 0);
+            for (int t = 0; t < NT; ++t) acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[t], a[mt], acc[mt][t], 0, 0, 0);
         if (more) {
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) a[mt] = na[mt];
@@ -274,25 +288,39 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
             for (int t = 0; t < NT; ++t) b[t] = nb[t];
         }
     }
+    // the MFMAs computed each tile transposed (weights as the A operand),
+    // so this lane holds channels co .. co+3 of pixel x: one 8-byte (BF16)
+    // or 16-byte (FP32) store each; out_stride, co_base and co are multiples
+    // of 4, so the stores are aligned
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + m;
-        const float bv = bias ? bias[co] : 0.0f;
+        const int co = t * 16 + 4 * kg;
+        float bv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bv[r] = bias ? bias[co + r] : 0.0f;
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
+            const int x = x0 + 16 * mt + m;
+            if (x >= W) continue;
+            float v[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int x = x0 + 16 * mt + 4 * kg + r;
-                if (x >= W) continue;
-                float v = acc[mt][t][r] + bv;
-                if (RELU) v = fmaxf(v, 0.0f);
-                const size_t o = (((size_t)n * H + y) * W + x) * out_stride + co_base + co;
-                if (OUT_F32) {
-                    float* of = static_cast<float*>(out);
-                    of[o] = ACCUM ? of[o] + v : v;
-                } else {
-                    static_cast<u16*>(out)[o] = f2bf(v);
+                v[r] = acc[mt][t][r] + bv[r];
+                if (RELU) v[r] = fmaxf(v[r], 0.0f);
+            }
+            const size_t o = (((size_t)n * H + y) * W + x) * out_stride + co_base + co;
+            if (OUT_F32) {
+                float4* of = reinterpret_cast<float4*>(static_cast<float*>(out) + o);
+                float4 w4 = make_float4(v[0], v[1], v[2], v[3]);
+                if (ACCUM) {
+                    const float4 c4 = *of;
+                    w4 = make_float4(c4.x + v[0], c4.y + v[1], c4.z + v[2], c4.w + v[3]);
                 }
+                *of = w4;
+            } else {
+                const uint2 h = make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                                           (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+                *reinterpret_cast<uint2*>(static_cast<u16*>(out) + o) = h;
             }
         }
     }
