@@ -86,6 +86,22 @@ def test_graph_render_equals_eager(ctx, tail):
         ctx.set_launch()
     assert st["passes"] == 6
     assert torch.equal(a, b)
+    # replays across calls with eager work and synchronisations between them:
+    # each graph-mode call captures its first pass and replays it twice; the
+    # image is checked against the eager one after every call
+    ctx.set_launch(tail_paths=tail)
+    try:
+        for i in range(4):
+            sc.render_device(cam, RenderSettings(spp=6, spp_per_pass=2, sample_offset=12 + 6 * i), a)
+            scratch = (a * 0.5 + 1.0).sum()  # unrelated eager work on the default stream
+            torch.cuda.synchronize()
+            sc.render_device(cam, RenderSettings(spp=6, spp_per_pass=2, sample_offset=12 + 6 * i, graph=True), b,
+                             stats=False)
+            ctx.synchronize()
+            assert torch.isfinite(scratch)
+            assert torch.equal(a, b), (tail, i)
+    finally:
+        ctx.set_launch()
 
 
 def test_pass_splits_do_not_change_images(ctx):
