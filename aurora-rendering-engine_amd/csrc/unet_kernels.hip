@@ -329,69 +329,110 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
 // ---------------------------------------------------- weight gradient ---
 // dW[co][tap][ci] = sum_p g[p][co] * X[p + tap][ci] over the rows of this
 // block: MFMA with M = 16 output channels, N = 16 input channels, K = 32
-// pixels of a row.  Each of the 4 waves walks every 4th row; the partial
-// tiles are summed in LDS and written per block (reduced afterwards in a
-// fixed order, so the result is deterministic).
+// pixels of a row.  A block owns one (co tile, ci tile) pair and all 9 taps:
+// per 32-pixel chunk a wave loads its g fragment once and, for each of the 3
+// input rows, a 10-pixel window of X from which the 3 column shifts are
+// taken in registers, then issues 9 MFMAs.  Each of the 4 waves walks every
+// 4th row; the partial tiles are summed in LDS and written per block and tap
+// (reduced afterwards in a fixed order, so the result is deterministic).
 __global__ __launch_bounds__(256) void k_conv3x3_dw(ConvIn in, const u16* __restrict__ g, int cout, int N, int H,
                                                     int W, int rows_per_block, float* __restrict__ partial) {
     __shared__ float red[4][256];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int cin = in.ca + in.cb;
     const int n_ci = cin / 16, n_co = cout / 16;
-    const int tile = blockIdx.y;  // (tap, co tile, ci tile)
-    const int tap = tile / (n_co * n_ci);
-    const int rem = tile % (n_co * n_ci);
-    const int co0 = (rem / n_ci) * 16, ci0 = (rem % n_ci) * 16;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int pair = blockIdx.y;  // (co tile, ci tile)
+    const int co0 = (pair / n_ci) * 16, ci0 = (pair % n_ci) * 16;
     const int m = lane & 15, kg = lane >> 4;
-    floatx4 acc = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    floatx4 acc[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc[k] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     const int r0 = blockIdx.x * rows_per_block;
     const int r1 = min(N * H, r0 + rows_per_block);
     for (int row = r0 + wave; row < r1; row += 4) {
         const int n = row / H, y = row % H;
         for (int x0 = 0; x0 < W; x0 += 32) {
-            Pack8 a, b;
+            const int xl = x0 + 8 * kg;  // this lane's first pixel of the chunk
+            Pack8 a;
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const int x = x0 + 8 * kg + j;
+                const int x = xl + j;
                 a.h[j] = x < W ? g[(((size_t)n * H + y) * W + x) * cout + co0 + m] : (u16)0;
-                b.h[j] = x < W ? gather1(in, n, y + dy, x + dx, H, W, ci0 + m) : (u16)0;
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc, 0, 0, 0);
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+                // X at pixels xl - 1 .. xl + 8 of row y + dy (zero outside the
+                // image; pixels past W meet a zero g value)
+                u16 win[10];
+#pragma unroll
+                for (int i = 0; i < 10; ++i) win[i] = gather1(in, n, y + dy, xl - 1 + i, H, W, ci0 + m);
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    Pack8 b;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) b.h[j] = win[j + dx + 1];
+                    acc[(dy + 1) * 3 + dx + 1] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[(dy + 1) * 3 + dx + 1], 0, 0, 0);
+                }
+            }
         }
     }
     // D[4*kg + r][m]: row = co offset, col = ci offset
+    const int ntiles = 9 * n_co * n_ci;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) red[wave][(4 * kg + r) * 16 + m] = acc[r];
-    __syncthreads();
-    const int t = threadIdx.x;
-    const float s = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-    partial[((size_t)blockIdx.x * gridDim.y + tile) * 256 + t] = s;
+    for (int tap = 0; tap < 9; ++tap) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[wave][(4 * kg + r) * 16 + m] = acc[tap][r];
+        __syncthreads();
+        const int t = threadIdx.x;
+        const float s = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+        partial[((size_t)blockIdx.x * ntiles + tap * (n_co * n_ci) + pair) * 256 + t] = s;
+        __syncthreads();  // red is rewritten for the next tap
+    }
 }
 
-// dW = sum over row blocks in order; scatter tiles into [Cout][9][Cin]
+// dW = sum over row blocks; scatter tiles into [Cout][9][Cin].  Eight
+// interleaved partial sums (block b into sum b mod 8) keep eight loads in
+// flight; they are combined in a fixed order, so the result is deterministic.
 __global__ void k_dw_reduce(const float* __restrict__ partial, int nblk, int ntiles, int cout, int cin,
                             float* __restrict__ dw) {
     const int idx = blockIdx.x * 256 + threadIdx.x;
     if (idx >= ntiles * 256) return;
     const int tile = idx / 256, e = idx % 256;
-    float s = 0.0f;
-    for (int b = 0; b < nblk; ++b) s += partial[((size_t)b * ntiles + tile) * 256 + e];
+    const size_t stride = (size_t)ntiles * 256;
+    const float* p = partial + (size_t)tile * 256 + e;
+    float q[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int b = 0;
+    for (; b + 8 <= nblk; b += 8) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] += p[(size_t)(b + k) * stride];
+    }
+    for (int k = 0; b < nblk; ++b, ++k) q[k] += p[(size_t)b * stride];
+    const float s = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
     const int n_ci = cin / 16, n_co = cout / 16;
     const int tap = tile / (n_co * n_ci), rem = tile % (n_co * n_ci);
     const int co = (rem / n_ci) * 16 + e / 16, ci = (rem % n_ci) * 16 + e % 16;
     dw[((size_t)co * 9 + tap) * cin + ci] = s;
 }
 
-// db[co] = sum_p g[p][co] (per-block partials, then an ordered sum)
+// db[co] = sum_p g[p][co] (per-block partials, then an ordered sum).  A
+// thread sums every groups-th pixel of the block's range into eight
+// interleaved partial sums (eight loads in flight), combined in a fixed order.
 __global__ void k_db_partial(const u16* __restrict__ g, int cout, size_t npix, size_t per_block,
                              float* __restrict__ partial) {
     __shared__ float red[256];
     const int co = threadIdx.x % cout, lane_group = threadIdx.x / cout, groups = 256 / cout;
-    float s = 0.0f;
+    float q[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     const size_t p0 = blockIdx.x * per_block, p1 = min(npix, p0 + per_block);
-    if (lane_group < groups)
-        for (size_t p = p0 + lane_group; p < p1; p += groups) s += bf2f(g[p * cout + co]);
+    if (lane_group < groups) {
+        size_t p = p0 + lane_group;
+        for (; p + 7 * (size_t)groups < p1; p += 8 * (size_t)groups) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] += bf2f(g[(p + (size_t)k * groups) * cout + co]);
+        }
+        for (int k = 0; p < p1; p += groups, ++k) q[k] += bf2f(g[p * cout + co]);
+    }
+    const float s = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
     red[threadIdx.x] = s;
     __syncthreads();
     if (threadIdx.x < cout) {
@@ -671,7 +712,7 @@ Status conv3x3_weight_grad(const ConvIn& in, const uint16_t* g, int cout, int N,
     const int cin = in.ca + in.cb;
     const int ntiles = 9 * (cout / 16) * (cin / 16);
     const int rows = N * H;
-    const int rows_per_block = 16;
+    const int rows_per_block = 4;  // one row per wave
     const int nblk = (rows + rows_per_block - 1) / rows_per_block;
     const size_t npix = (size_t)N * H * W;
     // bias gradient: 512 pixels per block (a 128x128x8 batch is 256 blocks)
@@ -680,7 +721,7 @@ Status conv3x3_weight_grad(const ConvIn& in, const uint16_t* g, int cout, int N,
     AUR_TRY(scratch.alloc(((size_t)nblk * ntiles * 256 + (size_t)nbb * cout) * 4));
     float* part = scratch.as<float>();
     float* dbpart = part + (size_t)nblk * ntiles * 256;
-    k_conv3x3_dw<<<dim3(nblk, ntiles), 256, 0, s>>>(in, g, cout, N, H, W, rows_per_block, part);
+    k_conv3x3_dw<<<dim3(nblk, ntiles / 9), 256, 0, s>>>(in, g, cout, N, H, W, rows_per_block, part);
     k_dw_reduce<<<nblocks((size_t)ntiles * 256), 256, 0, s>>>(part, nblk, ntiles, cout, cin, dw);
     k_db_partial<<<nbb, 256, 0, s>>>(g, cout, npix, per_block, dbpart);
     k_db_reduce<<<cout, 256, 0, s>>>(dbpart, nbb, cout, db);
