@@ -53,7 +53,7 @@ struct aurora_scene {
     aurora_ctx* ctx = nullptr;
     DevScene view{};
     uint32_t n_prims = 0;
-    DevBuf spheres, sphere_mat, quads, mats, texs, texels, perlin_vec, perlin_perm;
+    DevBuf spheres, sphere_mat, quads, quad_n, mats, texs, texels, perlin_vec, perlin_perm;
     DevBuf nodes, wide, morton, order;
     BvhScratch scr;
     bool built = false;
@@ -227,6 +227,15 @@ std::string camera_frame(const aurora_camera_desc& c, CamF& out) {
     out.W = c.image_width;
     out.H = c.image_height;
     return "";
+}
+
+// the unit normal of each quad, unitv(u x v): the same operations shading
+// would run per hit (each correctly rounded, no contraction), done once
+__global__ void k_quad_normals(const float4* quads, uint32_t n, float4* nrm) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const v3 w = unitv(cross(xyz(quads[3 * (size_t)i + 1]), xyz(quads[3 * (size_t)i + 2])));
+    nrm[i] = make_float4(w.x, w.y, w.z, 0.0f);
 }
 
 __global__ void k_pack3(const float* src, float4* dst, uint32_t n) {
@@ -872,6 +881,12 @@ int aurora_scene_upload(aurora_ctx* c, const aurora_scene_desc* d, aurora_scene*
     Status r = upload(s->spheres, d->spheres, (size_t)d->n_spheres * 16, st);
     if (!r.bad()) r = upload(s->sphere_mat, d->sphere_material, (size_t)d->n_spheres * 4, st);
     if (!r.bad()) r = upload(s->quads, q.data(), q.size() * 16, st);
+    if (!r.bad()) r = s->quad_n.alloc((size_t)std::max(1u, d->n_quads) * 16);
+    if (!r.bad() && d->n_quads) {
+        k_quad_normals<<<(d->n_quads + 255) / 256, 256, 0, st>>>(s->quads.as<float4>(), d->n_quads,
+                                                                  s->quad_n.as<float4>());
+        r = hip_status(hipGetLastError(), "k_quad_normals");
+    }
     if (!r.bad()) r = upload(s->mats, mats.data(), mats.size() * sizeof(aurora_material), st);
     if (!r.bad()) r = upload(s->texs, texs.data(), texs.size() * sizeof(aurora_texture), st);
     if (!r.bad()) r = upload(s->texels, d->texels, (size_t)d->n_texels * 12, st);
@@ -883,6 +898,7 @@ int aurora_scene_upload(aurora_ctx* c, const aurora_scene_desc* d, aurora_scene*
     v.spheres = s->spheres.as<float4>();
     v.sphere_mat = s->sphere_mat.as<uint32_t>();
     v.quads = s->quads.as<float4>();
+    v.quad_n = s->quad_n.as<float4>();
     v.mats = s->mats.as<aurora_material>();
     v.texs = s->texs.as<aurora_texture>();
     v.texels = s->texels.as<float>();
@@ -900,7 +916,7 @@ int aurora_scene_upload(aurora_ctx* c, const aurora_scene_desc* d, aurora_scene*
 void aurora_scene_destroy(aurora_scene* s) {
     if (!s) return;
     if (s->ctx && s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
-    DevBuf* bufs[] = {&s->spheres, &s->sphere_mat, &s->quads, &s->mats, &s->texs, &s->texels, &s->perlin_vec,
+    DevBuf* bufs[] = {&s->spheres, &s->sphere_mat, &s->quads, &s->quad_n, &s->mats, &s->texs, &s->texels, &s->perlin_vec,
                       &s->perlin_perm, &s->nodes, &s->wide, &s->morton, &s->order, &s->scr.box_lo, &s->scr.box_hi,
                       &s->scr.bounds, &s->scr.keys[0], &s->scr.keys[1], &s->scr.vals[0], &s->scr.vals[1],
                       &s->scr.hist, &s->scr.gamma, &s->scr.flags, &s->scr.int_parent, &s->scr.leaf_parent,

@@ -175,6 +175,7 @@ struct DevScene {
     const float4* spheres;        // (centre, radius)
     const uint32_t* sphere_mat;
     const float4* quads;          // 3 per quad: (Q, mat bits) (u, kind bits) (v, 0)
+    const float4* quad_n;         // per quad: (unitv(u x v), 0), computed once at upload
     const aurora_material* mats;
     const aurora_texture* texs;
     const float* texels;

@@ -176,7 +176,7 @@ __device__ __forceinline__ ShadeOut shade_core(const SceneRec& r, const ShadeBuf
         q0 = q[0];
         q1 = q[1];
         q2 = q[2];
-        outward = unitv(cross(xyz(q1), xyz(q2)));
+        outward = xyz(sc.quad_n[id - sc.n_spheres]);  // unitv(cross(u, v)), precomputed
         mi = __float_as_uint(q0.w);
     }
     const aurora_material m = sc.mats[mi];
