@@ -917,14 +917,11 @@ void aurora_scene_destroy(aurora_scene* s) {
     if (!s) return;
     if (s->ctx && s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
     DevBuf* bufs[] = {&s->spheres, &s->sphere_mat, &s->quads, &s->quad_n, &s->mats, &s->texs, &s->texels, &s->perlin_vec,
-                      &s->perlin_perm, &s->nodes, &s->wide, &s->morton, &s->order, &s->scr.box_lo, &s->scr.box_hi,
-                      &s->scr.bounds, &s->scr.keys[0], &s->scr.keys[1], &s->scr.vals[0], &s->scr.vals[1],
-                      &s->scr.hist, &s->scr.gamma, &s->scr.flags, &s->scr.int_parent, &s->scr.leaf_parent,
-                      &s->scr.int_pos, &s->scr.leaf_pos, &s->scr.arrive, &s->scr.depth, &s->scr.widx, &s->rec1,
-                      &s->tr_o, &s->tr_d,
+                      &s->perlin_perm, &s->nodes, &s->wide, &s->morton, &s->order, &s->rec1, &s->tr_o, &s->tr_d,
                       &s->tr_hit, &s->tr_spill, &s->tr_pool, &s->totals, &s->tr_key[0], &s->tr_key[1],
                       &s->tr_val[0], &s->tr_val[1], &s->tr_hist};
     for (DevBuf* b : bufs) b->release();
+    s->scr.release();
     delete s;
 }
 

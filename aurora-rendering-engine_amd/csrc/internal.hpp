@@ -65,10 +65,20 @@ struct DevBuf {
 struct BvhScratch {
     DevBuf box_lo, box_hi;      // float4 per prim (padded boxes)
     DevBuf bounds;              // 12 ordered-u32: scene lo/hi, centroid lo/hi
+    DevBuf bpart;               // 12 ordered-u32 per block of the bounds reduction
     DevBuf keys[2], vals[2];    // radix sort ping-pong
     DevBuf hist;                // 256 * blocks
-    DevBuf gamma, flags, int_parent, leaf_parent, int_pos, leaf_pos, arrive, depth;
+    DevBuf gamma, flags, int_parent, leaf_parent, int_pos, leaf_pos;
+    DevBuf jump[2];             // (ancestor, distance) per internal node, pointer-jumping ping-pong
+    DevBuf lvl_key, lvl_node;   // internal nodes sorted by depth
+    DevBuf lvl_start;           // first sorted index of each depth (257 u32)
     DevBuf widx;                // wide index of each internal node (exclusive scan of even-depth flags)
+    void release() {
+        DevBuf* all[] = {&box_lo, &box_hi, &bounds, &bpart, &keys[0], &keys[1], &vals[0], &vals[1], &hist, &gamma,
+                         &flags, &int_parent, &leaf_parent, &int_pos, &leaf_pos, &jump[0], &jump[1], &lvl_key,
+                         &lvl_node, &lvl_start, &widx};
+        for (DevBuf* b : all) b->release();
+    }
 };
 
 // Build the LBVH into `nodes` (2n float4 pairs) and the four-wide traversal

@@ -1,12 +1,18 @@
 // lbvh.hip — LBVH build on gfx950.
 //
-// Pipeline (one stream, no host round trips):
-//   k_bounds_init -> k_prim_bounds (padded primitive boxes + wave-reduced
-//   scene/centroid bounds, ordered-u32 atomics) -> k_morton (30-bit codes)
-//   -> 4 x {k_hist, scan, k_scatter} (stable LSD radix sort of (code, id)
-//   with 8-bit digits; the scatter ranks keys with wave64 ballots and LDS
-//   per-wave digit counts) -> k_karras (Karras 2012 radix tree) -> k_refit
-//   (bottom-up box union, one acquire/release arrival counter per node).
+// Pipeline (one stream, one host read of the per-depth node counts):
+//   k_prim_bounds (padded primitive boxes, per-block scene/centroid bounds)
+//   -> k_bounds_final (one block reduces the block bounds; no atomics)
+//   -> k_morton (30-bit codes) -> 4 x {k_hist, scan, k_scatter} (stable LSD
+//   radix sort of (code, id) with 8-bit digits; the scatter ranks keys with
+//   wave64 ballots and LDS per-wave digit counts) -> k_karras (Karras 2012
+//   radix tree) -> depth of every internal node by pointer jumping
+//   (k_jump_init, 7 x k_jump, k_jump_out) -> internal nodes sorted by depth
+//   (one radix pass) -> k_leaves, then one k_refit_level per depth, deepest
+//   first.  Each level reads child boxes written by earlier launches, so the
+//   refit needs no arrival counters: on gfx950 an agent-scope acquire or
+//   release writes back or invalidates the XCD's L2, and one pair per node
+//   cost 19 ms at 4M primitives.
 //
 // Layout: node 0 is the root, node 1 padding; the children of the internal
 // node with split gamma sit at 2+2*gamma and 3+2*gamma, so every child pair
@@ -39,52 +45,88 @@ __host__ __device__ __forceinline__ float o2f(uint32_t o) {
     return c.f;
 }
 
-__global__ void k_bounds_init(uint32_t* b) {
-    int t = threadIdx.x;
-    if (t < 12) b[t] = ((t / 3) % 2 == 0) ? 0xffffffffu : 0u;  // lo: min, hi: max
-}
-
-__device__ __forceinline__ float wave_min(float v) {
+__device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o));
     return v;
 }
-__device__ __forceinline__ float wave_max(float v) {
+__device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o));
     return v;
 }
 
-// bounds: [0..2] scene lo, [3..5] scene hi, [6..8] centroid lo, [9..11] hi
+// bounds: [0..2] scene lo, [3..5] scene hi, [6..8] centroid lo, [9..11] hi.
+// Reduced as ordered u32 (a total order: the result does not depend on the
+// grouping, -0 sorts below +0) in two stages: each block writes its 12
+// values to part, k_bounds_final reduces those.
+constexpr int kBoundBlocks = 1024;
+
+__device__ __forceinline__ bool is_lo(int k) { return ((k / 3) % 2) == 0; }
+
 __global__ __launch_bounds__(kBlock) void k_prim_bounds(DevScene sc, uint32_t n, float4* box_lo, float4* box_hi,
-                                                        uint32_t* bounds) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    float r[12];
+                                                        uint32_t* part) {
+    __shared__ uint32_t red[4][12];
+    uint32_t r[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) r[k] = ((k / 3) % 2 == 0) ? INFINITY : -INFINITY;
-    if (i < n) {
+    for (int k = 0; k < 12; ++k) r[k] = is_lo(k) ? 0xffffffffu : 0u;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         v3 lo, hi;
         prim_box(sc, i, lo, hi);
         box_lo[i] = make_float4(lo.x, lo.y, lo.z, 0.0f);
         box_hi[i] = make_float4(hi.x, hi.y, hi.z, 0.0f);
-        float l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+        const float l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            float c = (l[k] + h[k]) * 0.5f;
-            r[k] = l[k];
-            r[3 + k] = h[k];
-            r[6 + k] = c;
-            r[9 + k] = c;
+            const uint32_t c = f2o((l[k] + h[k]) * 0.5f);
+            r[k] = min(r[k], f2o(l[k]));
+            r[3 + k] = max(r[3 + k], f2o(h[k]));
+            r[6 + k] = min(r[6 + k], c);
+            r[9 + k] = max(r[9 + k], c);
         }
     }
 #pragma unroll
-    for (int k = 0; k < 12; ++k) r[k] = ((k / 3) % 2 == 0) ? wave_min(r[k]) : wave_max(r[k]);
+    for (int k = 0; k < 12; ++k) r[k] = is_lo(k) ? wave_umin(r[k]) : wave_umax(r[k]);
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
+        for (int k = 0; k < 12; ++k) red[wave][k] = r[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        uint32_t v = red[0][k];
+        for (int w = 1; w < kBlock / 64; ++w) v = is_lo(k) ? min(v, red[w][k]) : max(v, red[w][k]);
+        part[(size_t)blockIdx.x * 12 + k] = v;
+    }
+}
+
+// one block: thread t reduces the partials of blocks t, t + 256, ...
+__global__ __launch_bounds__(kBlock) void k_bounds_final(const uint32_t* part, uint32_t nparts, uint32_t* bounds) {
+    __shared__ uint32_t red[4][12];
+    uint32_t r[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) r[k] = is_lo(k) ? 0xffffffffu : 0u;
+    for (uint32_t b = threadIdx.x; b < nparts; b += kBlock) {
+#pragma unroll
         for (int k = 0; k < 12; ++k) {
-            if (((k / 3) % 2) == 0) atomicMin(&bounds[k], f2o(r[k]));
-            else atomicMax(&bounds[k], f2o(r[k]));
+            const uint32_t v = part[(size_t)b * 12 + k];
+            r[k] = is_lo(k) ? min(r[k], v) : max(r[k], v);
         }
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) r[k] = is_lo(k) ? wave_umin(r[k]) : wave_umax(r[k]);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) red[wave][k] = r[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        uint32_t v = red[0][k];
+        for (int w = 1; w < kBlock / 64; ++w) v = is_lo(k) ? min(v, red[w][k]) : max(v, red[w][k]);
+        bounds[k] = v;
     }
 }
 
@@ -328,16 +370,15 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, uint32_
     else { int_parent[ug + 1] = ui; int_pos[ug + 1] = 3 + 2 * ug; }
 }
 
-__global__ __launch_bounds__(kBlock) void k_fill(uint32_t* p, uint32_t n, uint32_t v) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) p[i] = v;
-}
+// ---- refit, one launch per depth ----
+// Every node record is written once: a leaf by k_leaves, an internal node by
+// the k_refit_level launch of its depth, after both of its children.
 
-__global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, const uint32_t* order, const float4* box_lo,
-                                                  const float4* box_hi, const uint32_t* leaf_pos,
-                                                  const uint32_t* leaf_parent, const uint32_t* int_pos,
-                                                  const uint32_t* int_parent, const uint32_t* gamma,
-                                                  uint32_t* arrive, float4* nodes) {
+// leaves: the primitive's box, its id, and the position of its parent
+__global__ __launch_bounds__(kBlock) void k_leaves(uint32_t n, const uint32_t* order, const float4* box_lo,
+                                                   const float4* box_hi, const uint32_t* leaf_pos,
+                                                   const uint32_t* leaf_parent, const uint32_t* int_pos,
+                                                   float4* nodes) {
     const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
     if (m >= n) return;
     const uint32_t prim = order[m];
@@ -347,56 +388,83 @@ __global__ __launch_bounds__(kBlock) void k_refit(uint32_t n, const uint32_t* or
         nodes[1] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(kNone));
         return;
     }
-    uint32_t par = leaf_parent[m];
     const uint32_t pos = leaf_pos[m];
-    nodes[2 * pos] = make_float4(blo.x, blo.y, blo.z, __uint_as_float(AURORA_NODE_LEAF | prim));
-    nodes[2 * pos + 1] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(int_pos[par]));
-    for (;;) {
-        // release this thread's node writes; the second arrival acquires both
-        uint32_t old = __hip_atomic_fetch_add(&arrive[par], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;
-        const uint32_t c = 2 + 2 * gamma[par];
-        const float4 a0 = nodes[2 * c], a1 = nodes[2 * c + 1], b0 = nodes[2 * c + 2], b1 = nodes[2 * c + 3];
-        const uint32_t ppos = int_pos[par];
-        const uint32_t gp = par == 0 ? kNone : int_pos[int_parent[par]];
-        nodes[2 * ppos] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), __uint_as_float(c));
-        nodes[2 * ppos + 1] =
-            make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), __uint_as_float(gp));
-        if (par == 0) return;
-        par = int_parent[par];
-    }
+    nodes[2 * (size_t)pos] = make_float4(blo.x, blo.y, blo.z, __uint_as_float(AURORA_NODE_LEAF | prim));
+    nodes[2 * (size_t)pos + 1] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(int_pos[leaf_parent[m]]));
 }
 
-__global__ __launch_bounds__(kBlock) void k_depth(uint32_t n, const uint32_t* leaf_parent,
-                                                  const uint32_t* int_parent, uint32_t* depth) {
-    const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
-    if (m >= n || n < 2) return;
-    uint32_t d = 1, p = leaf_parent[m];
-    while (p != 0 && p < n - 1 && d < 256) {
-        p = int_parent[p];
-        ++d;
-    }
-    atomicMax(depth, d);
+// the internal nodes of one depth: union of the two child records (written
+// by earlier launches), child index, parent position
+__global__ __launch_bounds__(kBlock) void k_refit_level(const uint32_t* lvl_node, uint32_t count,
+                                                        const uint32_t* gamma, const uint32_t* int_pos,
+                                                        const uint32_t* int_parent, float4* nodes) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t i = lvl_node[k];
+    const uint32_t c = 2 + 2 * gamma[i];
+    const float4 a0 = nodes[2 * (size_t)c], a1 = nodes[2 * (size_t)c + 1];
+    const float4 b0 = nodes[2 * (size_t)c + 2], b1 = nodes[2 * (size_t)c + 3];
+    const uint32_t gp = i == 0 ? kNone : int_pos[int_parent[i]];
+    const uint32_t pos = int_pos[i];
+    nodes[2 * (size_t)pos] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), __uint_as_float(c));
+    nodes[2 * (size_t)pos + 1] = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), __uint_as_float(gp));
 }
 
-// ---- four-wide traversal tree ----
-// flag[i] = 1 when internal node i has even depth (it becomes a wide node);
-// flag[ni] = 0, so the exclusive scan leaves the wide-node count at [ni]
-__global__ __launch_bounds__(kBlock) void k_wide_flags(uint32_t ni, const uint32_t* int_parent, uint32_t* f) {
+// ---- depth of every internal node by pointer jumping ----
+// J[i] = (ancestor a of i, number of edges from i up to a); kNone marks the
+// root's parent.  A round replaces (a, d) by (ancestor of a, d + its
+// distance), doubling the span, so after r rounds every node within 2^r
+// edges of the root is resolved.  A Karras tree over distinct 64-bit keys
+// (30-bit code, 32-bit index) is at most 64 internal levels deep: each
+// child's common prefix is longer than its parent's.  Seven rounds cover 128.
+constexpr int kJumpRounds = 7;
+
+__global__ __launch_bounds__(kBlock) void k_jump_init(uint32_t ni, const uint32_t* int_parent, uint2* J) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ni) return;
+    const uint32_t p = int_parent[i];
+    J[i] = make_uint2(p, p == kNone ? 0u : 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void k_jump(uint32_t ni, const uint2* A, uint2* B) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ni) return;
+    const uint2 a = A[i];
+    if (a.x == kNone) {
+        B[i] = a;
+        return;
+    }
+    const uint2 q = A[a.x];
+    B[i] = make_uint2(q.x, a.y + q.y);
+}
+
+// depth keys for the level sort (255 marks a node left unresolved, which the
+// host reports), node ids, and the four-wide flags: wide node where the
+// depth is even; flag[ni] = 0 so the exclusive scan leaves the count at [ni]
+__global__ __launch_bounds__(kBlock) void k_jump_out(uint32_t ni, const uint2* J, uint32_t* key, uint32_t* id,
+                                                     uint32_t* wflag) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i > ni) return;
     if (i == ni) {
-        f[i] = 0u;
+        wflag[i] = 0u;
         return;
     }
-    uint32_t d = 0, p = i;
-    while (p != 0 && p < ni && d < 256) {
-        p = int_parent[p];
-        ++d;
-    }
-    f[i] = (d & 1u) == 0 ? 1u : 0u;
+    const uint2 a = J[i];
+    const uint32_t d = (a.x == kNone && a.y < 255u) ? a.y : 255u;
+    key[i] = d;
+    id[i] = i;
+    wflag[i] = (d & 1u) == 0 ? 1u : 0u;
 }
 
+// first sorted index of each depth: the scanned digit-major histogram of the
+// one-pass sort holds it at hist[d * nblocks]
+__global__ void k_level_starts(const uint32_t* hist, uint32_t nblocks, uint32_t total, uint32_t* starts) {
+    const uint32_t d = threadIdx.x;
+    if (d < 256) starts[d] = hist[(size_t)d * nblocks];
+    if (d == 0) starts[256] = total;
+}
+
+// ---- four-wide traversal tree ----
 __device__ __forceinline__ void wide_slot(float4* w, int k, const float4* nodes, uint32_t pos, uint32_t ref) {
     const float4 lo = nodes[2 * (size_t)pos], hi = nodes[2 * (size_t)pos + 1];
     w[2 * k] = make_float4(lo.x, lo.y, lo.z, __uint_as_float(ref));
@@ -492,10 +560,12 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
     }
     const uint32_t n_nodes = n == 1 ? 1u : 2u * n;
     const uint32_t nb = blocks_for(n, kTile);
+    const uint32_t nbound = std::min<uint32_t>(blocks_for(n, kBlock), kBoundBlocks);
     AUR_TRY(nodes.alloc((size_t)n_nodes * 32));
     AUR_TRY(s.box_lo.alloc((size_t)n * 16));
     AUR_TRY(s.box_hi.alloc((size_t)n * 16));
     AUR_TRY(s.bounds.alloc(64));
+    AUR_TRY(s.bpart.alloc((size_t)kBoundBlocks * 12 * 4));
     for (int k = 0; k < 2; ++k) {
         AUR_TRY(s.keys[k].alloc((size_t)n * 4));
         AUR_TRY(s.vals[k].alloc((size_t)n * 4));
@@ -506,10 +576,13 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
     AUR_TRY(s.flags.alloc(ni * 4));
     AUR_TRY(s.int_parent.alloc(ni * 4));
     AUR_TRY(s.int_pos.alloc(ni * 4));
-    AUR_TRY(s.arrive.alloc(ni * 4));
     AUR_TRY(s.leaf_parent.alloc((size_t)n * 4));
     AUR_TRY(s.leaf_pos.alloc((size_t)n * 4));
-    AUR_TRY(s.depth.alloc(4));
+    AUR_TRY(s.jump[0].alloc(ni * 8));
+    AUR_TRY(s.jump[1].alloc(ni * 8));
+    AUR_TRY(s.lvl_key.alloc(ni * 4));
+    AUR_TRY(s.lvl_node.alloc(ni * 4));
+    AUR_TRY(s.lvl_start.alloc(257 * 4));
     const uint32_t n_flags = (uint32_t)ni + 1;
     AUR_TRY(s.widx.alloc(((size_t)n_flags + blocks_for(n_flags, kScanTile)) * 4));
     // the wide-node count is known only after the scan: room for its bound
@@ -520,9 +593,9 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
     AUR_HIP(hipEventCreate(&e0));
     AUR_HIP(hipEventCreate(&e1));
     AUR_HIP(hipEventRecord(e0, st));
-    k_bounds_init<<<1, 64, 0, st>>>(s.bounds.as<uint32_t>());
-    k_prim_bounds<<<blocks_for(n, kBlock), kBlock, 0, st>>>(sc, n, s.box_lo.as<float4>(), s.box_hi.as<float4>(),
-                                                            s.bounds.as<uint32_t>());
+    k_prim_bounds<<<nbound, kBlock, 0, st>>>(sc, n, s.box_lo.as<float4>(), s.box_hi.as<float4>(),
+                                             s.bpart.as<uint32_t>());
+    k_bounds_final<<<1, kBlock, 0, st>>>(s.bpart.as<uint32_t>(), nbound, s.bounds.as<uint32_t>());
     k_morton<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.box_lo.as<float4>(), s.box_hi.as<float4>(),
                                                        s.bounds.as<uint32_t>(), s.keys[0].as<uint32_t>(),
                                                        s.vals[0].as<uint32_t>());
@@ -534,30 +607,55 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
                                          s.vals[b].as<uint32_t>(), n, 8 * pass, s.hist.as<uint32_t>(), nb);
     }
     // four passes: sorted data is back in buffer 0
-    if (n > 1) {
-        k_karras<<<blocks_for(n - 1, kBlock), kBlock, 0, st>>>(
-            s.keys[0].as<uint32_t>(), n, s.gamma.as<uint32_t>(), s.flags.as<uint32_t>(), s.int_parent.as<uint32_t>(),
-            s.leaf_parent.as<uint32_t>(), s.int_pos.as<uint32_t>(), s.leaf_pos.as<uint32_t>(), nodes.as<float4>());
-        k_fill<<<blocks_for(n - 1, kBlock), kBlock, 0, st>>>(s.arrive.as<uint32_t>(), n - 1, 0u);
-    }
-    k_refit<<<blocks_for(n, kBlock), kBlock, 0, st>>>(
-        n, s.vals[0].as<uint32_t>(), s.box_lo.as<float4>(), s.box_hi.as<float4>(), s.leaf_pos.as<uint32_t>(),
-        s.leaf_parent.as<uint32_t>(), s.int_pos.as<uint32_t>(), s.int_parent.as<uint32_t>(), s.gamma.as<uint32_t>(),
-        s.arrive.as<uint32_t>(), nodes.as<float4>());
-    k_fill<<<1, kBlock, 0, st>>>(s.depth.as<uint32_t>(), 1, 0u);
-    k_depth<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.leaf_parent.as<uint32_t>(), s.int_parent.as<uint32_t>(),
-                                                      s.depth.as<uint32_t>());
-    // four-wide traversal tree
-    uint32_t n_wide = 1;
+    uint32_t depth = 0, n_wide = 1;
     if (n > 1) {
         const uint32_t nint = n - 1;
-        k_wide_flags<<<blocks_for(nint + 1, kBlock), kBlock, 0, st>>>(nint, s.int_parent.as<uint32_t>(),
-                                                                      s.widx.as<uint32_t>());
+        const uint32_t gi = blocks_for(nint, kBlock);
+        k_karras<<<gi, kBlock, 0, st>>>(s.keys[0].as<uint32_t>(), n, s.gamma.as<uint32_t>(), s.flags.as<uint32_t>(),
+                                        s.int_parent.as<uint32_t>(), s.leaf_parent.as<uint32_t>(),
+                                        s.int_pos.as<uint32_t>(), s.leaf_pos.as<uint32_t>(), nodes.as<float4>());
+        uint2* J[2] = {s.jump[0].as<uint2>(), s.jump[1].as<uint2>()};
+        k_jump_init<<<gi, kBlock, 0, st>>>(nint, s.int_parent.as<uint32_t>(), J[0]);
+        for (int r = 0; r < kJumpRounds; ++r) k_jump<<<gi, kBlock, 0, st>>>(nint, J[r & 1], J[(r + 1) & 1]);
+        k_jump_out<<<blocks_for(nint + 1, kBlock), kBlock, 0, st>>>(nint, J[kJumpRounds & 1],
+                                                                    s.keys[1].as<uint32_t>(),
+                                                                    s.vals[1].as<uint32_t>(), s.widx.as<uint32_t>());
+        // one 8-bit pass: the sorted pairs land in (lvl_key, lvl_node)
+        AUR_TRY(radix_sort_pairs(s.keys[1].as<uint32_t>(), s.vals[1].as<uint32_t>(), s.lvl_key.as<uint32_t>(),
+                                 s.lvl_node.as<uint32_t>(), s.hist.as<uint32_t>(), nint, 1, st));
+        k_level_starts<<<1, 256, 0, st>>>(s.hist.as<uint32_t>(), blocks_for(nint, kTile), nint,
+                                          s.lvl_start.as<uint32_t>());
+        uint32_t starts[257];
+        AUR_HIP(hipMemcpyAsync(starts, s.lvl_start.p, sizeof(starts), hipMemcpyDeviceToHost, st));
+        AUR_HIP(hipStreamSynchronize(st));
+        if (starts[255] != nint) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return Status{AURORA_ERR_RUNTIME, "LBVH build: node depths did not resolve"};
+        }
+        uint32_t maxd = 0;
+        for (uint32_t d = 0; d < 255; ++d)
+            if (starts[d + 1] > starts[d]) maxd = d;
+        depth = maxd + 1;  // a deepest internal node has two leaf children
+        k_leaves<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.vals[0].as<uint32_t>(), s.box_lo.as<float4>(),
+                                                           s.box_hi.as<float4>(), s.leaf_pos.as<uint32_t>(),
+                                                           s.leaf_parent.as<uint32_t>(), s.int_pos.as<uint32_t>(),
+                                                           nodes.as<float4>());
+        for (int d = (int)maxd; d >= 0; --d) {
+            const uint32_t cnt = starts[d + 1] - starts[d];
+            if (cnt == 0) continue;
+            k_refit_level<<<blocks_for(cnt, kBlock), kBlock, 0, st>>>(
+                s.lvl_node.as<uint32_t>() + starts[d], cnt, s.gamma.as<uint32_t>(), s.int_pos.as<uint32_t>(),
+                s.int_parent.as<uint32_t>(), nodes.as<float4>());
+        }
+        // four-wide traversal tree from the even-depth flags written above
         scan_exclusive(s.widx.as<uint32_t>(), nint + 1, st);
-        k_wide_fill<<<blocks_for(nint, kBlock), kBlock, 0, st>>>(nint, s.gamma.as<uint32_t>(), s.flags.as<uint32_t>(),
-                                                                 s.widx.as<uint32_t>(), nodes.as<float4>(),
-                                                                 wide.as<float4>());
+        k_wide_fill<<<gi, kBlock, 0, st>>>(nint, s.gamma.as<uint32_t>(), s.flags.as<uint32_t>(),
+                                           s.widx.as<uint32_t>(), nodes.as<float4>(), wide.as<float4>());
     } else {
+        k_leaves<<<1, kBlock, 0, st>>>(n, s.vals[0].as<uint32_t>(), s.box_lo.as<float4>(), s.box_hi.as<float4>(),
+                                       s.leaf_pos.as<uint32_t>(), s.leaf_parent.as<uint32_t>(),
+                                       s.int_pos.as<uint32_t>(), nodes.as<float4>());
         k_wide_single<<<1, 64, 0, st>>>(nodes.as<float4>(), wide.as<float4>());
     }
     AUR_HIP(hipGetLastError());
@@ -565,9 +663,8 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
     std::swap(morton, s.keys[0]);
     std::swap(order, s.vals[0]);
 
-    uint32_t b[12], depth = 0;
+    uint32_t b[12];
     AUR_HIP(hipMemcpyAsync(b, s.bounds.p, 48, hipMemcpyDeviceToHost, st));
-    AUR_HIP(hipMemcpyAsync(&depth, s.depth.p, 4, hipMemcpyDeviceToHost, st));
     if (n > 1) AUR_HIP(hipMemcpyAsync(&n_wide, s.widx.as<uint32_t>() + (n - 1), 4, hipMemcpyDeviceToHost, st));
     AUR_HIP(hipStreamSynchronize(st));
     float ms = 0.0f;

@@ -663,7 +663,6 @@ Status conv3x3_forward(const ConvIn& in, const uint16_t* wk_all, int kpad, const
                        bool out_f32, bool accum, bool relu, int N, int H, int W, int cout, hipStream_t s) {
     if ((in.ca % 16) || (in.cb % 16) || (cout % 16) || cout < 16 || (in.ca + in.cb) < 16)
         return Status{AURORA_ERR_INVALID, "conv3x3: channel counts must be multiples of 16"};
-    const int cin = in.ca + in.cb;
     // the LDS-tiled kernel where its tiles fit 64 KB (every layer but the
     // level-2 decoder input, 64 + 64 channels, which gathers from L1/L2)
     const int tr = lds_tile_bytes(in.ca, in.cb) <= 65536 ? 4 : 0;
