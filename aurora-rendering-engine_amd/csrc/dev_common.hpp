@@ -217,6 +217,30 @@ __device__ __forceinline__ bool hit_sphere(float4 sp, v3 o, v3 d, float tmin, fl
     return true;
 }
 
+// hit_sphere for the traversals with its early returns folded into one
+// predicate (same operations, same decisions); the far root is still
+// computed only by lanes that need it, which most waves skip entirely.
+__device__ __forceinline__ bool hit_sphere_t(float4 sp, v3 o, v3 d, float tmin, float tmax, float& t) {
+    const v3 oc = V(sp.x, sp.y, sp.z) - o;
+    const float a = dot(d, d);
+    const float h = dot(d, oc);
+    const float rr = sp.w * sp.w;
+    const float c = dot(oc, oc) - rr;
+    const v3 x = cross(d, oc);
+    const float disc = a * rr - dot(x, x);
+    const float q = h + copysignf(sqrtf(disc), h);
+    const bool cand = (disc >= 0.0f) & !(q == 0.0f);
+    const bool pos = q > 0.0f;
+    float root = (pos ? c : q) / (pos ? q : a);
+    bool in = (root > tmin) & (root < tmax);
+    if (cand & !in) {
+        root = (pos ? q : c) / (pos ? a : q);
+        in = (root > tmin) & (root < tmax);
+    }
+    t = root;
+    return cand & in;
+}
+
 // Möller–Trumbore on the (Q, u, v) record: alpha multiplies u, beta v;
 // inclusive edges as in src/object/triangle.cpp:98-111.
 __device__ __forceinline__ bool hit_quad(float4 q0, float4 q1, float4 q2, v3 o, v3 d, float tmin, float tmax,
@@ -240,6 +264,29 @@ __device__ __forceinline__ bool hit_quad(float4 q0, float4 q1, float4 q2, v3 o, 
     al = alpha;
     be = beta;
     return true;
+}
+
+// hit_quad's test for the traversals, without branches: the same operations
+// and the same accept/reject decisions (every comparison written as in
+// hit_quad, so NaNs decide alike), all evaluated.  A branch per early return
+// costs exec-mask saves and restores on every lane and saves VALU work only
+// when no lane of the wave goes on.
+__device__ __forceinline__ bool hit_quad_t(float4 q0, float4 q1, float4 q2, v3 o, v3 d, float tmin, float tmax,
+                                           float& t) {
+    const v3 u = xyz(q1), v = xyz(q2);
+    const v3 h = cross(d, v);
+    const float a = dot(u, h);
+    const float f = 1.0f / a;
+    const v3 s = o - xyz(q0);
+    const float alpha = f * dot(s, h);
+    const v3 qv = cross(s, u);
+    const float beta = f * dot(d, qv);
+    const bool tri = __float_as_uint(q1.w) == AURORA_QUAD_TRIANGLE;
+    const float tt = f * dot(v, qv);
+    const bool edge = tri ? (alpha + beta > 1.0f) : (beta > 1.0f);
+    t = tt;
+    return !(fabsf(a) < 1e-20f) & !(alpha < 0.0f) & !(alpha > 1.0f) & !(beta < 0.0f) & !edge & (tt > tmin) &
+           (tt < tmax);
 }
 
 // slab test against (tmin, tcur]; fminf/fmaxf are IEEE minNum/maxNum.

@@ -76,23 +76,24 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
 
     // te: the entry distance of the primitive's box; a hit before it does not
     // count (this makes the closest hit independent of the visit order)
+    // (an id past the primitives, which no well-formed tree holds, reads
+    // primitive 0 and is neither counted nor taken)
     auto test_prim = [&](uint32_t id, float te) {
-        float t;
+        const bool valid = id < n_prims;
+        const uint32_t k = valid ? id : 0u;
+        float t = 0.0f;
         bool h;
-        if (id >= n_prims) return;
-        if (id < n_spheres) {
-            if (COUNT) ++cs;
-            h = hit_sphere(ldg4(S, id), o, d, tmin, tmax, t);
+        if (k < n_spheres) {
+            if (COUNT) cs += valid ? 1u : 0u;
+            h = hit_sphere_t(ldg4(S, k), o, d, tmin, tmax, t);
         } else {
-            if (COUNT) ++cq;
-            const size_t q = 3 * (size_t)(id - n_spheres);
-            float al, be;
-            h = hit_quad(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t, al, be);
+            if (COUNT) cq += valid ? 1u : 0u;
+            const size_t q = 3 * (size_t)(k - n_spheres);
+            h = hit_quad_t(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t);
         }
-        if (h && t >= te && (t < best_t || (t == best_t && id < best_id))) {
-            best_t = t;
-            best_id = id;
-        }
+        const bool take = valid & h & (t >= te) & ((t < best_t) | ((t == best_t) & (id < best_id)));
+        best_t = take ? t : best_t;
+        best_id = take ? id : best_id;
     };
     auto finish = [&]() {
         const bool miss = best_id == kNoPrim;

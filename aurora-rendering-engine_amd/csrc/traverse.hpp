@@ -56,10 +56,11 @@ __device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3
     const uint32_t f0 = __float_as_uint(l0.w), f1 = __float_as_uint(l1.w);
     const uint32_t f2 = __float_as_uint(l2.w), f3 = __float_as_uint(l3.w);
     float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
-    bool e0 = f0 != AURORA_BVH4_EMPTY && slab(l0, h0, o, inv, tmin, tcur, t0);
-    bool e1 = f1 != AURORA_BVH4_EMPTY && slab(l1, h1, o, inv, tmin, tcur, t1);
-    bool e2 = f2 != AURORA_BVH4_EMPTY && slab(l2, h2, o, inv, tmin, tcur, t2);
-    bool e3 = f3 != AURORA_BVH4_EMPTY && slab(l3, h3, o, inv, tmin, tcur, t3);
+    // all four slabs evaluated (an empty slot's box is zeros), no branches
+    bool e0 = slab(l0, h0, o, inv, tmin, tcur, t0) & (f0 != AURORA_BVH4_EMPTY);
+    bool e1 = slab(l1, h1, o, inv, tmin, tcur, t1) & (f1 != AURORA_BVH4_EMPTY);
+    bool e2 = slab(l2, h2, o, inv, tmin, tcur, t2) & (f2 != AURORA_BVH4_EMPTY);
+    bool e3 = slab(l3, h3, o, inv, tmin, tcur, t3) & (f3 != AURORA_BVH4_EMPTY);
     // the hit leaf slots, compacted in slot order with their entry
     // distances: the wave runs the j-th primitive test only when some lane
     // has more than j leaves, instead of once per slot that any lane hit
@@ -67,18 +68,18 @@ __device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3
     float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f, n3 = 0.0f;
     int np = 0;
     auto add = [&](bool& e, uint32_t f, float tn) {
-        if (!(e && (f & AURORA_NODE_LEAF))) return;
+        const bool lf = e & ((f & AURORA_NODE_LEAF) != 0u);
         const uint32_t id = f & ~AURORA_NODE_LEAF;
-        p0 = np == 0 ? id : p0;
-        p1 = np == 1 ? id : p1;
-        p2 = np == 2 ? id : p2;
-        p3 = np == 3 ? id : p3;
-        n0 = np == 0 ? tn : n0;
-        n1 = np == 1 ? tn : n1;
-        n2 = np == 2 ? tn : n2;
-        n3 = np == 3 ? tn : n3;
-        ++np;
-        e = false;
+        p0 = (lf & (np == 0)) ? id : p0;
+        p1 = (lf & (np == 1)) ? id : p1;
+        p2 = (lf & (np == 2)) ? id : p2;
+        p3 = (lf & (np == 3)) ? id : p3;
+        n0 = (lf & (np == 0)) ? tn : n0;
+        n1 = (lf & (np == 1)) ? tn : n1;
+        n2 = (lf & (np == 2)) ? tn : n2;
+        n3 = (lf & (np == 3)) ? tn : n3;
+        np += lf ? 1 : 0;
+        e = e & !lf;
     };
     add(e0, f0, t0);
     add(e1, f1, t1);
@@ -120,21 +121,22 @@ __device__ __forceinline__ void traverse_closest_wide(const DevScene& sc, v3 o, 
     float best_t = tmax;
     uint32_t best_id = 0x7fffffffu;
     // tn: the entry distance of the primitive's box; hits before it do not count
+    // (an id past the primitives, which no well-formed tree holds, reads
+    // primitive 0 and is never taken)
     auto test_prim = [&](uint32_t id, float tn) {
-        if (id >= n_prims) return;
-        float t;
+        const bool valid = id < n_prims;
+        const uint32_t k = valid ? id : 0u;
+        float t = 0.0f;
         bool h;
-        if (id < n_spheres) {
-            h = hit_sphere(ldg4(S, id), o, d, tmin, tmax, t);
+        if (k < n_spheres) {
+            h = hit_sphere_t(ldg4(S, k), o, d, tmin, tmax, t);
         } else {
-            const size_t q = 3 * (size_t)(id - n_spheres);
-            float al, be;
-            h = hit_quad(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t, al, be);
+            const size_t q = 3 * (size_t)(k - n_spheres);
+            h = hit_quad_t(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t);
         }
-        if (h && t >= tn && (t < best_t || (t == best_t && id < best_id))) {
-            best_t = t;
-            best_id = id;
-        }
+        const bool take = valid & h & (t >= tn) & ((t < best_t) | ((t == best_t) & (id < best_id)));
+        best_t = take ? t : best_t;
+        best_id = take ? id : best_id;
     };
     uint32_t cur = 0;
     int sp = 0;
@@ -185,21 +187,22 @@ __device__ __forceinline__ void traverse_closest_binary(const DevScene& sc, v3 o
     float best_t = tmax;
     uint32_t best_id = 0x7fffffffu;
     // tn: the entry distance of the primitive's box; hits before it do not count
+    // (an id past the primitives, which no well-formed tree holds, reads
+    // primitive 0 and is never taken)
     auto test_prim = [&](uint32_t id, float tn) {
-        if (id >= n_prims) return;
-        float t;
+        const bool valid = id < n_prims;
+        const uint32_t k = valid ? id : 0u;
+        float t = 0.0f;
         bool h;
-        if (id < n_spheres) {
-            h = hit_sphere(ldg4(S, id), o, d, tmin, tmax, t);
+        if (k < n_spheres) {
+            h = hit_sphere_t(ldg4(S, k), o, d, tmin, tmax, t);
         } else {
-            const size_t q = 3 * (size_t)(id - n_spheres);
-            float al, be;
-            h = hit_quad(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t, al, be);
+            const size_t q = 3 * (size_t)(k - n_spheres);
+            h = hit_quad_t(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t);
         }
-        if (h && t >= tn && (t < best_t || (t == best_t && id < best_id))) {
-            best_t = t;
-            best_id = id;
-        }
+        const bool take = valid & h & (t >= tn) & ((t < best_t) | ((t == best_t) & (id < best_id)));
+        best_t = take ? t : best_t;
+        best_id = take ? id : best_id;
     };
     if (n_nodes > 0) {
         const float4 r0 = ldg4(N, 0), r1 = ldg4(N, 1);
