@@ -138,18 +138,24 @@ struct ShadeOut {
     bool alive;
 };
 
-// One shading event of path `path` at segment `depth`: background or light
-// emission on termination (written once to L[path]), else scatter.
+// The pixel and sample of path id `path` (sample-major within its scene).
+__device__ __forceinline__ void path_pixel(const SceneRec& r, uint32_t path, uint32_t& pixel, uint32_t& sample) {
+    const uint32_t local = path - r.path_off;
+    const uint32_t s_local = local / r.WH;
+    pixel = local - s_local * r.WH;
+    sample = r.sample_base + s_local;
+}
+
+// One shading event of path `path` (at `pixel`, `sample`) at segment
+// `depth`: background or light emission on termination (written once to
+// L[path]), else scatter.
 __device__ __forceinline__ ShadeOut shade_core(const SceneRec& r, const ShadeBufs& b, v3 org, v3 dir, v3 beta,
-                                               float t, int prim, uint32_t path, uint32_t depth) {
+                                               float t, int prim, uint32_t path, uint32_t pixel, uint32_t sample,
+                                               uint32_t depth) {
     const DevScene& sc = r.sc;
     const bool aov = r.aov && depth == 0;
     ShadeOut out;
     out.alive = false;
-    const uint32_t local = path - r.path_off;
-    const uint32_t s_local = local / r.WH;
-    const uint32_t pixel = local - s_local * r.WH;
-    const uint32_t sample = r.sample_base + s_local;
     const v3 bg = V(r.bg[0], r.bg[1], r.bg[2]);
     if (prim < 0) {
         const v3 e = beta * bg;
@@ -261,7 +267,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t
             const float2 H = b.hit[i];
             path = __float_as_uint(O.w);
             sid = ONE ? 0u : __float_as_uint(D.w);
-            o = shade_core(pa.recs[sid], b, xyz(O), xyz(D), xyz(B), H.x, __float_as_int(H.y), path, depth);
+            const SceneRec& r = pa.recs[sid];
+            uint32_t pixel, sample;
+            path_pixel(r, path, pixel, sample);
+            o = shade_core(r, b, xyz(O), xyz(D), xyz(B), H.x, __float_as_int(H.y), path, pixel, sample, depth);
             ++segs;
         }
         const uint64_t m = __ballot(o.alive);
@@ -322,7 +331,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     uint32_t* pool = b.counters + kPoolBase + kPoolStride * part;
     uint32_t b_next = 0, b_end = 0;  // wave-uniform, offsets within the range
     bool has = false, done = false;
-    uint32_t path = 0, sid = 0, depth = 0;
+    uint32_t path = 0, sid = 0, depth = 0, pixel = 0, sample = 0;  // pixel, sample: once per path
     v3 org = V(0, 0, 0), dir = V(0, 0, 0), beta = V(0, 0, 0);
     unsigned long long segs = 0;
     for (;;) {
@@ -365,6 +374,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
                     dir = xyz(D);
                     beta = xyz(B);
                     depth = depth0;
+                    path_pixel(pa.recs[sid], path, pixel, sample);
                     has = true;
                 }
             }
@@ -383,7 +393,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         else
             traverse_closest_binary(r.sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill,
                                     spill_stride, t, prim);
-        const ShadeOut o = shade_core(r, b, org, dir, beta, t, prim, path, depth);
+        const ShadeOut o = shade_core(r, b, org, dir, beta, t, prim, path, pixel, sample, depth);
         if (!o.alive) {
             has = false;
         } else {
