@@ -121,7 +121,7 @@ TAIL_ALWAYS = (1 << 32) - 1  # fused-tail switch above any queue: one persistent
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain 1 GiB copy
 # gives), keyed (kernel, width, tail switch, four-wide render tree); the
 # summaries and commands are in the profiles/ directory.
-PMC_TRAFFIC = {("k_tail", 1024, TAIL_ALWAYS, True): 670.6e6}  # profiles/tail_cornell_1024x1024.md
+PMC_TRAFFIC = {("k_tail", 1024, TAIL_ALWAYS, True): 667.8e6}  # profiles/tail_cornell_1024x1024.md
 
 
 def note(msg):
