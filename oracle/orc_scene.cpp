@@ -201,13 +201,22 @@ static uint32_t quantise(float c, float lo, float hi) {
     return uint32_t(q);
 }
 
-// Four-wide traversal tree.  Every internal binary node of even depth
-// becomes one wide node, numbered in Karras index order; its slots are the
-// expansions of its two children in order (a leaf child is itself, an
-// internal child contributes its two children).  A slot that is an internal
-// binary node refers to that node's wide index (its depth is even again).
+// Four-wide traversal tree.  A wide node is made for the root and for every
+// internal binary node that a wide node's slots refer to, numbered in Karras
+// index order.  Its slots are a cut below the binary node: for scenes of at
+// least kWideGreedyMin primitives, greedy by surface area (start with the two
+// children; while fewer than four, the internal slot with the largest box
+// surface area, dx*dy + dy*dz + dz*dx in float, the first in slot order on
+// ties, is replaced in place by its two children); for smaller scenes,
+// pairwise (each internal child replaced by its two children, so the wide
+// nodes are the even-depth binary nodes).  A leaf slot refers to its
+// primitive, an internal slot to that node's wide index.  (The GPU builds
+// the same tree: aurora lbvh.hip wide_cut, k_wide_mark, k_wide_fill.)  No
+// reference counterpart.
+static constexpr uint32_t kWideGreedyMin = 4096;
+
 static void collapse_wide(Scene& s, const uint32_t* gamma, const uint8_t* lleaf, const uint8_t* rleaf,
-                          const uint32_t* idepth, uint32_t ni) {
+                          uint32_t ni) {
     s.wide.clear();
     const uint32_t n = s.n_prims();
     auto slot_of = [&](uint32_t pos, uint32_t ref) {
@@ -228,33 +237,74 @@ static void collapse_wide(Scene& s, const uint32_t* gamma, const uint8_t* lleaf,
         s.stats.n_wide = 1;
         return;
     }
+    // a slot: binary node position, leaf or not, internal index if not
+    struct Ref { uint32_t pos; bool leaf; uint32_t j; };
+    auto area = [&](uint32_t pos) {
+        const aurora_bvh_node& nd = s.nodes[pos];
+        const float dx = nd.hi[0] - nd.lo[0], dy = nd.hi[1] - nd.lo[1], dz = nd.hi[2] - nd.lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    const bool greedy = n >= kWideGreedyMin;
+    // children of internal i: positions 2+2g (internal index g) and 3+2g (g+1)
+    auto cut = [&](uint32_t i, Ref* out) {
+        const uint32_t g = gamma[i];
+        if (!greedy) {
+            int k = 0;
+            for (uint32_t c = 0; c < 2; ++c) {
+                const uint32_t j = g + c;
+                if (c == 0 ? lleaf[i] : rleaf[i]) {
+                    out[k++] = {2 + 2 * g + c, true, j};
+                    continue;
+                }
+                const uint32_t gj = gamma[j];
+                out[k++] = {2 + 2 * gj, lleaf[j] != 0, gj};
+                out[k++] = {3 + 2 * gj, rleaf[j] != 0, gj + 1};
+            }
+            return k;
+        }
+        out[0] = {2 + 2 * g, lleaf[i] != 0, g};
+        out[1] = {3 + 2 * g, rleaf[i] != 0, g + 1};
+        int k = 2;
+        while (k < 4) {
+            int best = -1;
+            float ba = 0.0f;
+            for (int q = 0; q < k; ++q)
+                if (!out[q].leaf) {
+                    const float a = area(out[q].pos);
+                    if (best < 0 || a > ba) { best = q; ba = a; }
+                }
+            if (best < 0) break;
+            const uint32_t j = out[best].j, gj = gamma[j];
+            for (int q = k; q > best + 1; --q) out[q] = out[q - 1];
+            out[best] = {2 + 2 * gj, lleaf[j] != 0, gj};
+            out[best + 1] = {3 + 2 * gj, rleaf[j] != 0, gj + 1};
+            ++k;
+        }
+        return k;
+    };
+    std::vector<uint8_t> root(ni, 0);
+    root[0] = 1;
+    std::vector<uint32_t> st{0};
+    Ref r[4];
+    while (!st.empty()) {
+        const uint32_t i = st.back();
+        st.pop_back();
+        const int k = cut(i, r);
+        for (int q = 0; q < k; ++q)
+            if (!r[q].leaf) { root[r[q].j] = 1; st.push_back(r[q].j); }
+    }
     std::vector<uint32_t> widx(ni, 0xffffffffu);
     uint32_t nw = 0;
     for (uint32_t i = 0; i < ni; ++i)
-        if ((idepth[i] & 1u) == 0) widx[i] = nw++;
+        if (root[i]) widx[i] = nw++;
     s.wide.assign(nw, aurora_bvh4_node{});
     for (uint32_t i = 0; i < ni; ++i) {
-        if (widx[i] == 0xffffffffu) continue;
+        if (!root[i]) continue;
+        const int k = cut(i, r);
         aurora_bvh4_node& w = s.wide[widx[i]];
-        int k = 0;
-        const uint32_t g = gamma[i];
-        // children of internal i: positions 2+2g (internal index g) and 3+2g (g+1)
-        for (int c = 0; c < 2; ++c) {
-            const uint32_t pos = 2 + 2 * g + c;
-            const bool leaf = c == 0 ? lleaf[i] : rleaf[i];
-            if (leaf) {
-                w.slot[k++] = slot_of(pos, s.nodes[pos].child);
-                continue;
-            }
-            const uint32_t j = g + c;  // internal index of the child
-            const uint32_t gj = gamma[j];
-            for (int e = 0; e < 2; ++e) {
-                const uint32_t gpos = 2 + 2 * gj + e;
-                const bool gleaf = e == 0 ? lleaf[j] : rleaf[j];
-                w.slot[k++] = slot_of(gpos, gleaf ? s.nodes[gpos].child : widx[gj + e]);
-            }
-        }
-        for (; k < 4; ++k) w.slot[k] = empty;
+        for (int q = 0; q < k; ++q)
+            w.slot[q] = slot_of(r[q].pos, r[q].leaf ? s.nodes[r[q].pos].child : widx[r[q].j]);
+        for (int q = k; q < 4; ++q) w.slot[q] = empty;
     }
     s.stats.n_wide = nw;
 }
@@ -318,7 +368,7 @@ void build_lbvh(Scene& s) {
     if (n == 1) {
         s.nodes[0] = leaf_node(0, 0xffffffffu);
         s.stats.depth = 0;
-        collapse_wide(s, nullptr, nullptr, nullptr, nullptr, 0);
+        collapse_wide(s, nullptr, nullptr, nullptr, 0);
         return;
     }
 
@@ -388,7 +438,7 @@ void build_lbvh(Scene& s) {
         nd.parent = i == 0 ? 0xffffffffu : int_pos[int_parent[i]];
     }
     s.stats.depth = maxdepth;
-    collapse_wide(s, gamma.data(), lleaf.data(), rleaf.data(), idepth.data(), ni);
+    collapse_wide(s, gamma.data(), lleaf.data(), rleaf.data(), ni);
 }
 
 // ----------------------------------------------------- primitive tests ---

@@ -49,7 +49,7 @@ def test_lbvh_bit_exact(ctx, name):
     assert np.array_equal(g.wide(), o.wide())
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 100])
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 100, 2048])
 def test_lbvh_small_and_duplicates(ctx, n):
     say("lbvh small", n)
     rng = np.random.default_rng(n)

@@ -72,26 +72,76 @@ def check_topology(nodes, n, lo, hi):
     return depth
 
 
-def even_depth_internal_count(nodes):
-    """Internal binary nodes at even depth, from the parent links."""
+WIDE_GREEDY_MIN = 4096  # primitives from which the build collapses greedily
+
+
+def expected_wide(nodes, greedy):
+    """The four-wide tree restated from the binary node array alone: the
+    root and every internal node some wide node refers to get a wide node,
+    numbered in internal (Karras) index order, which the node positions give
+    (internal node j sits at position 2j+2 or 2j+1; the root at 0).  Greedy:
+    slots start as the two children and the internal slot of largest float32
+    surface area (the first on ties) is opened in place until there are four
+    or only leaves.  Pairwise: each internal child is opened once.  Returns
+    the expected wide array (uint32, 8 words per slot)."""
     child = nodes[:, 3]
-    count, stack = 0, [(0, 0)]
+    f = nodes.view(np.float32)
+    lo, hi = f[:, 0:3], f[:, 4:7]
+
+    def index(pos):
+        return 0 if pos == 0 else (pos - 2) // 2 if pos % 2 == 0 else (pos - 1) // 2
+
+    def area(pos):
+        dx, dy, dz = (hi[pos] - lo[pos]).astype(np.float32)
+        return dx * dy + dy * dz + dz * dx  # float32 scalars: one rounding per operation
+
+    def is_leaf(pos):
+        return bool(int(child[pos]) & LEAF)
+
+    def cut(pos):
+        c = int(child[pos])
+        slots = [c, c + 1]
+        if not greedy:
+            return [q for sp in slots for q in ([sp] if is_leaf(sp) else [int(child[sp]), int(child[sp]) + 1])]
+        while len(slots) < 4:
+            best, ba = -1, None
+            for q, sp in enumerate(slots):
+                if not is_leaf(sp):
+                    a = area(sp)
+                    if best < 0 or a > ba:
+                        best, ba = q, a
+            if best < 0:
+                break
+            cc = int(child[slots[best]])
+            slots[best:best + 1] = [cc, cc + 1]
+        return slots
+
+    roots, stack = [], [0]
     while stack:
-        i, d = stack.pop()
-        c = int(child[i])
-        if c & LEAF:
-            continue
-        count += d % 2 == 0
-        stack += [(c, d + 1), (c + 1, d + 1)]
-    return count
+        pos = stack.pop()
+        roots.append(pos)
+        stack += [sp for sp in cut(pos) if not is_leaf(sp)]
+    roots.sort(key=index)
+    widx = {pos: k for k, pos in enumerate(roots)}
+    out = np.zeros((len(roots), 4, 8), np.uint32)
+    for k, pos in enumerate(roots):
+        for q, sp in enumerate(cut(pos)):
+            out[k, q, 0:3] = nodes[sp, 0:3]
+            out[k, q, 4:7] = nodes[sp, 4:7]
+            out[k, q, 3] = child[sp] if is_leaf(sp) else widx[sp]
+        for q in range(len(cut(pos)), 4):
+            out[k, q, 3] = NONE
+    return out.reshape(len(roots), 32)
 
 
 def check_wide(wide, nodes, n, st):
     """Structural invariants of the four-wide tree: every primitive once as
     a leaf slot, every wide node but the root referenced once, valid slots
     first, a referenced node's slot boxes contained in the referring slot
-    and exactly covering it, the root covering the scene, and one wide node
-    per even-depth internal binary node."""
+    and exactly covering it, the root covering the scene, fewer than four
+    slots only where every candidate is a leaf, and the whole array equal to
+    the collapse (greedy or pairwise by scene size) restated from the binary
+    nodes."""
     assert len(wide) == st["n_wide"]
     if n == 0:
         assert len(wide) == 0
@@ -121,7 +171,7 @@ def check_wide(wide, nodes, n, st):
     assert np.array_equal(slo[0][valid[0]].min(0), np.array(st["scene_lo"], np.float32))
     assert np.array_equal(shi[0][valid[0]].max(0), np.array(st["scene_hi"], np.float32))
     if n > 1:
-        assert len(wide) == even_depth_internal_count(nodes)
+        assert np.array_equal(wide.reshape(len(wide), 32), expected_wide(nodes, n >= WIDE_GREEDY_MIN))
 
 
 def build(arr):
@@ -160,8 +210,10 @@ def spheres_only(centres, r=0.1):
     return b.arrays()
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 7, 64])
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 4096])
 def test_tiny_and_duplicate_scenes(n):
+    """Random and coincident spheres (4096 of them: the greedy collapse, with
+    every surface area tied in the coincident scene)."""
     rng = np.random.default_rng(n)
     for arr in (spheres_only(rng.uniform(-1, 1, (n, 3))), spheres_only(np.zeros((n, 3)))):
         sc, st = build(arr)
