@@ -126,10 +126,12 @@ typedef struct aurora_bvh_node {
 } aurora_bvh_node;
 
 /* Four-wide node of the traversal tree (128 bytes, one cache line): the
- * binary LBVH collapsed so that each node holds the grandchildren of an
- * even-depth binary node (a child that is a leaf stays as itself).  ref is
- * AURORA_NODE_LEAF | primitive, the index of another four-wide node, or
- * AURORA_BVH4_EMPTY.  Node 0 is the root. */
+ * binary LBVH collapsed so that each node holds a cut of up to four nodes
+ * below a binary node.  Scenes of at least 4096 primitives: greedy (the
+ * internal slot of largest surface area opened first); smaller scenes:
+ * pairwise (each internal child opened once, so the wide nodes are the
+ * even-depth binary nodes).  ref is AURORA_NODE_LEAF | primitive, the index
+ * of another four-wide node, or AURORA_BVH4_EMPTY.  Node 0 is the root. */
 #define AURORA_BVH4_EMPTY 0xffffffffu
 typedef struct aurora_bvh4_slot {
     float lo[3];
