@@ -153,6 +153,36 @@ def test_closest_hit_2_24_rays_against_brute_force(ctx, name):
     assert np.array_equal(bt[sub].cpu().numpy().view(np.uint32), ot.view(np.uint32))
 
 
+def test_full_size_stress_tree_and_hits(ctx):
+    """The traversal stress scene at its full size (2M spheres + 2M
+    triangles, 4M leaves, the greedy four-wide collapse): the GPU LBVH equals
+    the oracle's word for word (order, binary nodes, four-wide nodes, stats),
+    and the closest hits of 16384 incoherent rays over it (ids, t bits and
+    per-ray fetch counts) equal the oracle's."""
+    say("full-size stress tree")
+    arr = scenes.stress_scene()
+    g, gs, o, os_ = both(ctx, arr)
+    for k in ("n_prims", "n_nodes", "n_wide", "depth", "scene_lo", "scene_hi", "centroid_lo", "centroid_hi"):
+        assert gs[k] == os_[k], k
+    assert gs["n_prims"] == 4_000_000
+    assert np.array_equal(g.order()[1], o.order()[1])
+    assert np.array_equal(g.nodes(), o.nodes())
+    assert np.array_equal(g.wide(), o.wide())
+    org, dirs = scenes.random_rays(16384, [-0.05] * 3, [1.05] * 3, seed=77)
+    say("full-size stress hits")
+    op, ot, oc = o.trace(org, dirs, tmin=1e-4, counts=True, threads=16)
+    n = len(org)
+    dp = torch.zeros(n, dtype=torch.int32, device="cuda")
+    dt = torch.zeros(n, dtype=torch.float32, device="cuda")
+    cnt = torch.zeros((n, 3), dtype=torch.int32, device="cuda")
+    g.trace_device(dev(org), dev(dirs), dp, dt, tmin=1e-4, counts=cnt)
+    ctx.synchronize()
+    assert (op >= 0).mean() > 0.99
+    assert np.array_equal(dp.cpu().numpy(), op)
+    assert np.array_equal(dt.cpu().numpy().view(np.uint32), ot.view(np.uint32))
+    assert np.array_equal(cnt.cpu().numpy().astype(np.uint32), oc)
+
+
 def test_closest_hit_on_silhouettes_independent_of_tree(ctx):
     """Rays grazing small distant spheres among crossing quads, where a
     primitive's computed hit can fall outside its box: GPU queries over the
