@@ -82,6 +82,10 @@ template <int NT, bool RELU, bool OUT_F32, bool ACCUM>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvIn in, const u16* __restrict__ wk, int kpad,
                                                  const float* __restrict__ bias, void* out, int N, int H, int W,
                                                  int out_stride, int co_base) {
+    // blockIdx.y: the group of 16*NT output channels
+    co_base += blockIdx.y * 16 * NT;
+    wk += (size_t)blockIdx.y * 16 * NT * kpad;
+    if (bias) bias += blockIdx.y * 16 * NT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int segs = (W + 63) / 64;
     const int seg = blockIdx.x * 4 + wave;
@@ -192,6 +196,10 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
                                                      const float* __restrict__ bias, void* out, int N, int H, int W,
                                                      int out_stride, int co_base) {
     extern __shared__ __align__(16) unsigned char tile[];
+    // blockIdx.y: the group of 16*NT output channels
+    co_base += blockIdx.y * 16 * NT;
+    wk += (size_t)blockIdx.y * 16 * NT * kpad;
+    if (bias) bias += blockIdx.y * 16 * NT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int segs = (W + 63) / 64, bands = (H + 3) / 4;
     const int x0 = (blockIdx.x % segs) * 64;
@@ -669,20 +677,31 @@ Status conv3x3_forward(const ConvIn& in, const uint16_t* wk_all, int kpad, const
     const unsigned grid = (unsigned)(((size_t)N * H * ((W + 63) / 64) + 3) / 4);  // 4 row segments per block
     const unsigned grid_lds = tr ? (unsigned)((size_t)N * ((H + 3) / 4) * ((W + 63) / 64)) : 0u;
     const size_t lds = tr ? (size_t)lds_tile_bytes(in.ca, in.cb) : 0;
-    // output channels in groups of at most 64 (4 MFMA tiles per wave)
-    for (int co_base = 0; co_base < cout; co_base += 64) {
-    const int nt = std::min(64, cout - co_base) / 16;
+    // Output channels: waves of up to 64 channels (4 MFMA tiles, every A
+    // fragment feeding four MFMAs), one launch per 64.  A layer of fewer than
+    // 128 pixel blocks in the LDS form (the coarsest level: 64 blocks at
+    // 1024^2) instead takes one 16-channel group per grid row (y), so its one
+    // launch has cout/16 times the blocks: 22.8 -> 16.8 us per layer.  At 256
+    // blocks the wide waves measured faster (21 vs 29 us), and in the gather
+    // form far faster (44 vs 81 us: the L1 gathers are no longer shared by
+    // four tiles).
+    const unsigned pix_blocks = tr ? grid_lds : grid;
+    const bool narrow = tr && pix_blocks < 128 && cout > 16;
+    const int step = narrow ? cout : 64;
+    for (int co_base = 0; co_base < cout; co_base += step) {
+    const int nt = narrow ? 1 : std::min(64, cout - co_base) / 16;
+    const unsigned groups = narrow ? (unsigned)(cout / 16) : 1u;
     const uint16_t* wk = wk_all + (size_t)co_base * kpad;
     const float* bias = bias_all ? bias_all + co_base : nullptr;
 #define AUR_CONV(NTv, R, F, A)                                                                                  \
     if (tr && in.cb)                                                                                           \
-        k_conv3x3_lds<NTv, R, F, A, true><<<grid_lds, 64 * tr, lds, s>>>(in, wk, kpad, bias, out, N, H, W, cout,   \
-                                                                        co_base);                              \
+        k_conv3x3_lds<NTv, R, F, A, true><<<dim3(grid_lds, groups), 64 * tr, lds, s>>>(in, wk, kpad, bias, out, N, \
+                                                                                      H, W, cout, co_base);     \
     else if (tr)                                                                                               \
-        k_conv3x3_lds<NTv, R, F, A, false><<<grid_lds, 64 * tr, lds, s>>>(in, wk, kpad, bias, out, N, H, W, cout,  \
-                                                                         co_base);                             \
+        k_conv3x3_lds<NTv, R, F, A, false><<<dim3(grid_lds, groups), 64 * tr, lds, s>>>(in, wk, kpad, bias, out,   \
+                                                                                       N, H, W, cout, co_base); \
     else                                                                                                       \
-        k_conv3x3<NTv, R, F, A><<<grid, 256, 0, s>>>(in, wk, kpad, bias, out, N, H, W, cout, co_base)
+        k_conv3x3<NTv, R, F, A><<<dim3(grid, groups), 256, 0, s>>>(in, wk, kpad, bias, out, N, H, W, cout, co_base)
 #define AUR_CONV_NT(R, F, A)          \
     switch (nt) {                     \
         case 1: AUR_CONV(1, R, F, A); break; \
