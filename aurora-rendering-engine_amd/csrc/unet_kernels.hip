@@ -671,9 +671,10 @@ Status conv3x3_forward(const ConvIn& in, const uint16_t* wk_all, int kpad, const
                        bool out_f32, bool accum, bool relu, int N, int H, int W, int cout, hipStream_t s) {
     if ((in.ca % 16) || (in.cb % 16) || (cout % 16) || cout < 16 || (in.ca + in.cb) < 16)
         return Status{AURORA_ERR_INVALID, "conv3x3: channel counts must be multiples of 16"};
-    // the LDS-tiled kernel where its tiles fit 64 KB (every layer but the
-    // level-2 decoder input, 64 + 64 channels, which gathers from L1/L2)
-    const int tr = lds_tile_bytes(in.ca, in.cb) <= 65536 ? 4 : 0;
+    // the LDS-tiled kernel where its tiles fit 80 KB (of the 160 KB of LDS per
+    // CU; every layer of the network: the largest, the level-2 decoder input
+    // of 64 + 64 channels, takes 76.6 KB), else the gather form
+    const int tr = lds_tile_bytes(in.ca, in.cb) <= 81920 ? 4 : 0;
     const unsigned grid = (unsigned)(((size_t)N * H * ((W + 63) / 64) + 3) / 4);  // 4 row segments per block
     const unsigned grid_lds = tr ? (unsigned)((size_t)N * ((H + 3) / 4) * ((W + 63) / 64)) : 0u;
     const size_t lds = tr ? (size_t)lds_tile_bytes(in.ca, in.cb) : 0;
