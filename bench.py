@@ -119,9 +119,10 @@ TAIL_ALWAYS = (1 << 32) - 1  # fused-tail switch above any queue: one persistent
 
 # HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain 1 GiB copy
-# gives), keyed (kernel, width, tail switch, four-wide render tree); the
-# summaries and commands are in the profiles/ directory.
-PMC_TRAFFIC = {("k_tail", 1024, TAIL_ALWAYS, True): 667.8e6}  # profiles/tail_cornell_1024x1024.md
+# gives), keyed (kernel, width, samples per step, tail switch, four-wide
+# render tree); the summaries and commands are in the profiles/ directory.
+PMC_TRAFFIC = {("k_tail", 1024, 8, TAIL_ALWAYS, True): 667.8e6,  # profiles/tail_cornell_1024x1024.md
+               ("k_tail", 1024, 32, TAIL_ALWAYS, True): 2694.2e6}
 
 
 def note(msg):
@@ -301,7 +302,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--width", type=int, default=1024)
-    ap.add_argument("--spp-per-step", type=int, default=8)
+    # one step renders the 32-spp image that the denoised path of the
+    # time-to-relMSE metric turns into its final frame
+    ap.add_argument("--spp-per-step", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tail-paths", type=int, default=None, help="fused-tail switch (default: library's)")
     ap.add_argument("--sweep-tail", action="store_true", help="time several fused-tail switch points")
@@ -436,7 +439,7 @@ def main():
         kernel, launches, k_ms = "k_extend", max(1, first["extend_launches"]), first["extend_ms"]
         alg_bytes = alg_trav
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-    traffic = PMC_TRAFFIC.get((kernel, W, shape.get("tail_paths"), bool(shape.get("wide"))))
+    traffic = PMC_TRAFFIC.get((kernel, W, S, shape.get("tail_paths"), bool(shape.get("wide"))))
     out = {
         "metric": METRIC,
         "value": paths / elapsed,
@@ -466,7 +469,8 @@ def main():
     if traffic:
         out["roofline"]["note"] = (f"measured HBM traffic {traffic / 1e9:.2f} GB per launch is "
                                    f"{traffic / alg_bytes:.1%} of the algorithmic bytes: the tree and primitive "
-                                   "records are L1/L2 hits, so this kernel is bound by instruction issue, not HBM")
+                                   "records are L1/L2 hits, so this kernel is bound by instruction issue, not HBM, "
+                                   "and the algorithmic rate can exceed the HBM peak")
     if not args.no_metric:
         out["time_to_relmse"] = time_to_relmse_section(ctx, scene, cam)
     if not args.no_stress:
