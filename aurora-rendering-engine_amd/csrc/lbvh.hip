@@ -496,32 +496,44 @@ __device__ __forceinline__ float box_area(const float4* nodes, uint32_t pos) {
 }
 
 // the slots of internal node i (children of internal i: positions 2+2g,
-// internal index g, and 3+2g, index g+1; flags bit 0/1: left/right is a leaf)
+// internal index g, and 3+2g, index g+1; flags bit 0/1: left/right is a
+// leaf).  Every array index below is a constant after unrolling, so the cut
+// stays in registers.
 __device__ __forceinline__ int wide_cut(uint32_t i, const uint32_t* gamma, const uint32_t* flags,
                                         const float4* nodes, bool greedy, WideRef* out) {
     const uint32_t g = gamma[i], fl = flags[i];
     if (!greedy) {
         int k = 0;
+#pragma unroll
         for (uint32_t c = 0; c < 2; ++c) {
             const uint32_t j = g + c;
-            if (fl & (1u << c)) {
-                out[k++] = WideRef{2 + 2 * g + c, j, true};
-                continue;
+            const bool leaf = (fl & (1u << c)) != 0u;
+            // a leaf's j may be one past the internal nodes: read at g instead
+            const uint32_t jr = leaf ? g : j;
+            const uint32_t gj = gamma[jr], fj = leaf ? 0u : flags[jr];
+            // a leaf child stays itself; an internal child gives its two children
+            const WideRef first = leaf ? WideRef{2 + 2 * g + c, j, true} : WideRef{2 + 2 * gj, gj, (fj & 1u) != 0u};
+            const WideRef second = WideRef{3 + 2 * gj, gj + 1, (fj & 2u) != 0u};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q == k) out[q] = first;
+                if (!leaf && q == k + 1) out[q] = second;
             }
-            const uint32_t gj = gamma[j], fj = flags[j];
-            out[k++] = WideRef{2 + 2 * gj, gj, (fj & 1u) != 0u};
-            out[k++] = WideRef{3 + 2 * gj, gj + 1, (fj & 2u) != 0u};
+            k += leaf ? 1 : 2;
         }
         return k;
     }
     out[0] = WideRef{2 + 2 * g, g, (fl & 1u) != 0u};
     out[1] = WideRef{3 + 2 * g, g + 1, (fl & 2u) != 0u};
+    out[2] = out[3] = WideRef{0u, 0u, true};
     int k = 2;
-    while (k < 4) {
+#pragma unroll
+    for (int round = 0; round < 2; ++round) {
         int best = -1;
         float ba = 0.0f;
-        for (int q = 0; q < k; ++q)
-            if (!out[q].leaf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < k && !out[q].leaf) {
                 const float a = box_area(nodes, out[q].pos);
                 if (best < 0 || a > ba) {
                     best = q;
@@ -529,10 +541,21 @@ __device__ __forceinline__ int wide_cut(uint32_t i, const uint32_t* gamma, const
                 }
             }
         if (best < 0) break;
-        const uint32_t j = out[best].j, gj = gamma[j], fj = flags[j];
-        for (int q = k; q > best + 1; --q) out[q] = out[q - 1];
-        out[best] = WideRef{2 + 2 * gj, gj, (fj & 1u) != 0u};
-        out[best + 1] = WideRef{3 + 2 * gj, gj + 1, (fj & 2u) != 0u};
+        uint32_t j = out[0].j;
+#pragma unroll
+        for (int q = 1; q < 4; ++q) j = best == q ? out[q].j : j;
+        const uint32_t gj = gamma[j], fj = flags[j];
+        const WideRef lo = WideRef{2 + 2 * gj, gj, (fj & 1u) != 0u};
+        const WideRef hi = WideRef{3 + 2 * gj, gj + 1, (fj & 2u) != 0u};
+        // slot best becomes (lo, hi); the slots after it move up one
+#pragma unroll
+        for (int q = 3; q >= 1; --q) {
+            if (q > best + 1) out[q] = out[q - 1];
+            if (q == best + 1) out[q] = hi;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q == best) out[q] = lo;
         ++k;
     }
     return k;
