@@ -306,9 +306,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t
 // wave is not held by its longest path.  Same arithmetic per path as the
 // extend/shade pair, so results are unchanged.
 constexpr int kTailRefill = 4;
-// paths a wave claims per pool atomic: 128 measured 1.5% faster than 64 on
-// the Cornell box (fewer claims on the shared counters) and 7% faster than 32
-constexpr uint32_t kTailBatch = 128;
+// Paths a wave claims per pool atomic: about a sixteenth of a wave's share
+// of the pass, within [64, 128].  Large passes take 128 (Cornell, 33.5 M
+// paths: 1.5% faster than 64, fewer claims on the shared counters; 32 was 7%
+// slower); small passes of long paths keep 64, which balances the waves'
+// last claims (a 2 M-path stress render lost 5-15% at 128).
+constexpr uint32_t kTailBatchMin = 64, kTailBatchMax = 128;
 
 // five waves per SIMD: the register budget this leaves (84) holds the kernel
 // without spills; the resident grid (five blocks per CU) matches it.  WIDE:
@@ -332,6 +335,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     uint32_t r_n = (uint32_t)(((uint64_t)n * (part + 1)) / parts) - r_lo;
     uint32_t* pool = b.counters + kPoolBase + kPoolStride * part;
     uint32_t b_next = 0, b_end = 0;  // wave-uniform, offsets within the range
+    const uint32_t batch = __builtin_amdgcn_readfirstlane(
+        min(kTailBatchMax, max(kTailBatchMin, n / (gridDim.x * (kBlock / 64) * 16u))));
     bool has = false, done = false;
     uint32_t path = 0, sid = 0, depth = 0, pixel = 0, sample = 0;  // pixel, sample: once per path
     v3 org = V(0, 0, 0), dir = V(0, 0, 0), beta = V(0, 0, 0);
@@ -346,7 +351,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
                 uint32_t base = r_n;
                 if (lane == 0) {
                     if (__hip_atomic_load(pool, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < r_n)
-                        base = atomicAdd(pool, kTailBatch);
+                        base = atomicAdd(pool, batch);
                 }
                 // every lane of the wave is active here (no lane leaves the
                 // loop before the whole wave does), so lane 0 is the first
@@ -354,7 +359,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
                 base = __builtin_amdgcn_readfirstlane(base);
                 if (base < r_n) {
                     b_next = base;
-                    b_end = min(r_n, base + kTailBatch);
+                    b_end = min(r_n, base + batch);
                 } else if (++spent < parts) {
                     part = part + 1 == parts ? 0u : part + 1;
                     r_lo = (uint32_t)(((uint64_t)n * part) / parts);
