@@ -89,7 +89,11 @@ Status lbvh_build(const DevScene& sc, uint32_t n_prims, DevBuf& nodes, DevBuf& w
 
 // Stable LSD radix sort of (key, value) pairs, 8 bits per pass from bit 0.
 // Sorted pairs end in (k0, v0) for an even number of passes, else in
-// (k1, v1).  hist holds radix_hist_bytes(n).
+// (k1, v1).  hist holds radix_hist_bytes(n); after the call it holds the
+// last pass's exclusive scan in digit-major order (entry d * blocks + b is
+// where block b's keys of digit d start, blocks = ceil(n / 2048)), so
+// hist[d * blocks] is the first sorted index of digit d (the LBVH build reads
+// its per-depth node ranges there).
 size_t radix_hist_bytes(uint32_t n);
 Status radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t* hist, uint32_t n,
                         int passes, hipStream_t stream);
