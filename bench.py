@@ -1,9 +1,11 @@
 """Benchmark: paths/s of the wavefront path tracer on the Cornell box at
 1024x1024 (max depth 50, Philox seed 0) on one MI355X.
 
-A step is one wavefront pass of S samples per pixel over the whole image
-(raygen, up to 50 x {extend, shade}, accumulate), with the scene, LBVH and
-accumulation buffer already resident in HBM.  Prints one JSON line.
+A step is one pass of S samples per pixel over the whole image (32 by
+default: raygen; the bounces, in the persistent fused tail or as extend/shade
+kernels per bounce, whichever launch shape the autotuner measures faster;
+accumulate), with the scene, LBVH and accumulation buffer already resident in
+HBM.  Prints one JSON line.
 """
 import argparse
 import json

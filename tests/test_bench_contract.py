@@ -1,0 +1,32 @@
+"""bench.py's contract, checked on the CPU: a run on more than one GPU is
+refused with a message, and the table of measured HBM traffic covers the
+default Cornell step, so the default run reports roofline.traffic."""
+import importlib.util
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BENCH = os.path.join(ROOT, "bench.py")
+
+
+def load_bench():
+    spec = importlib.util.spec_from_file_location("bench_under_test", BENCH)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, BENCH, "--gpus", "2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_traffic_table_covers_default_step():
+    b = load_bench()
+    # the default step: 1024^2, 32 spp, the whole pass in the persistent tail
+    # over the four-wide tree (the shape the autotuner picks on this scene)
+    key = ("k_tail", 1024, 32, b.TAIL_ALWAYS, True)
+    assert key in b.PMC_TRAFFIC and b.PMC_TRAFFIC[key] > 0
+    assert b.HBM_PEAK_GBS == 8000.0
