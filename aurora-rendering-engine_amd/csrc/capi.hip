@@ -321,12 +321,12 @@ Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n,
             AUR_TRY(s->tr_hist.alloc(radix_hist_bytes(n)));
             a.order = s->tr_val[0].as<uint32_t>();
         }
-        hipEvent_t e0 = nullptr, e1 = nullptr;
+        EventPair ev;
         if (ts) {
-            AUR_HIP(hipEventCreate(&e0));
-            AUR_HIP(hipEventCreate(&e1));
-            AUR_HIP(hipEventRecord(e0, c->stream));
+            AUR_TRY(ev.create());
+            AUR_HIP(hipEventRecord(ev.e[0], c->stream));
         }
+        hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
         Status ls;
         if (sort) {
             ls = launch_ray_keys(a.ray_o, a.ray_d, n, s->view.nodes, s->tr_key[0].as<uint32_t>(),
@@ -342,8 +342,6 @@ Status trace_device(aurora_scene* s, const float* o, const float* d, uint32_t n,
             if (!es.bad()) es = hip_status(hipEventSynchronize(e1), "hipEventSynchronize");
             float ms = 0.0f;
             if (!es.bad()) es = hip_status(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
             AUR_TRY(ls);
             AUR_TRY(es);
             *ts = aurora_trace_stats{};

@@ -61,6 +61,19 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Two timing events, destroyed on every return path.
+struct EventPair {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    Status create() {
+        for (hipEvent_t& x : e) AUR_HIP(hipEventCreate(&x));
+        return Status::ok();
+    }
+    ~EventPair() {
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
 // Scratch of the LBVH build.
 struct BvhScratch {
     DevBuf box_lo, box_hi;      // float4 per prim (padded boxes)

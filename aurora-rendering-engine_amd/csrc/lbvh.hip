@@ -688,9 +688,9 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
     // (every internal node)
     AUR_TRY(wide.alloc(ni * 128));
 
-    hipEvent_t e0, e1;
-    AUR_HIP(hipEventCreate(&e0));
-    AUR_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    AUR_TRY(ev.create());
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     AUR_HIP(hipEventRecord(e0, st));
     k_prim_bounds<<<nbound, kBlock, 0, st>>>(sc, n, s.box_lo.as<float4>(), s.box_hi.as<float4>(),
                                              s.bpart.as<uint32_t>());
@@ -727,11 +727,7 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
         uint32_t starts[257];
         AUR_HIP(hipMemcpyAsync(starts, s.lvl_start.p, sizeof(starts), hipMemcpyDeviceToHost, st));
         AUR_HIP(hipStreamSynchronize(st));
-        if (starts[255] != nint) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            return Status{AURORA_ERR_RUNTIME, "LBVH build: node depths did not resolve"};
-        }
+        if (starts[255] != nint) return Status{AURORA_ERR_RUNTIME, "LBVH build: node depths did not resolve"};
         uint32_t maxd = 0;
         for (uint32_t d = 0; d < 255; ++d)
             if (starts[d + 1] > starts[d]) maxd = d;
@@ -777,8 +773,6 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
     AUR_HIP(hipStreamSynchronize(st));
     float ms = 0.0f;
     AUR_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     local.n_nodes = n_nodes;
     local.n_wide = n_wide;
     local.depth = depth;
