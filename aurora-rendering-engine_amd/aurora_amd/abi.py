@@ -6,6 +6,8 @@ sizes against the C compiler's (tests/test_abi.py).
 import ctypes as C
 import os
 
+ABI_VERSION = 3  # AURORA_ABI_VERSION of the header this mirror follows
+
 AURORA_OK = 0
 AURORA_ERR_INVALID = 1
 AURORA_ERR_RUNTIME = 2
@@ -21,6 +23,9 @@ MAT_DIELECTRIC = 2
 MAT_DIFFUSE_LIGHT = 3
 
 BVH4_EMPTY = 0xFFFFFFFF
+BVH4_PAIRWISE = 0
+BVH4_GREEDY = 1
+TRAVERSAL_STACK = 96  # entries of the device traversal stack
 TEX_SOLID = 0
 TEX_CHECKER = 1
 TEX_NOISE = 2
@@ -31,6 +36,10 @@ RENDER_COUNT_FETCHES = 2
 RENDER_GRAPH = 4
 LOSS_LOG_L2 = 0
 LOSS_REL_L2 = 1
+
+TRANSFER_GAMMA2 = 0
+TRANSFER_GAMMA22 = 1
+TRANSFER_LINEAR = 2
 
 NODE_LEAF = 0x80000000
 MISS = -1
@@ -82,6 +91,8 @@ class BvhStats(C.Structure):
         ("n_nodes", C.c_uint32),
         ("depth", C.c_uint32),
         ("n_wide", C.c_uint32),
+        ("wide_depth", C.c_uint32),
+        ("wide_form", C.c_uint32),
         ("scene_lo", C.c_float * 3),
         ("scene_hi", C.c_float * 3),
         ("centroid_lo", C.c_float * 3),
@@ -175,7 +186,10 @@ EXPORTS = [
      [_vp, C.POINTER(CameraDesc), C.POINTER(RenderDesc), _vp, _vp, _vp, C.POINTER(RenderStats)]),
     ("aurora_render_device", C.c_int,
      [_vp, C.POINTER(CameraDesc), C.POINTER(RenderDesc), _vp, _vp, _vp, C.POINTER(RenderStats)]),
-    ("aurora_resolve_rgb8", C.c_int, [_vp, _u32, _u32, _u32, _vp]),
+    ("aurora_camera_rays", C.c_int, [_vp, C.POINTER(CameraDesc), _u32, _vp, _vp, _u32, _vp, _vp]),
+    ("aurora_camera_rays_device", C.c_int, [_vp, C.POINTER(CameraDesc), _u32, _vp, _vp, _u32, _vp, _vp]),
+    ("aurora_resolve_rgb8", C.c_int, [_vp, _u32, _u32, _u32, _u32, _vp]),
+    ("aurora_resolve_rgb8_device", C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _vp]),
     ("aurora_render_batch_device", C.c_int, [_vp, C.POINTER(BatchItem), _u32, _u32, C.POINTER(RenderStats)]),
     ("aurora_denoiser_create", C.c_int, [_vp, _u32, C.POINTER(_vp)]),
     ("aurora_denoiser_destroy", None, [_vp]),
@@ -217,6 +231,9 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.aurora_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{p} has ABI version {lib.aurora_abi_version()}, this binding expects {ABI_VERSION} "
+                           "(rebuild with __graft_entry__.build())")
     if path is None:
         _lib = lib
     return lib

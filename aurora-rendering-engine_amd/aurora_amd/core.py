@@ -199,6 +199,34 @@ class Context:
     def upload(self, arrays: SceneArrays):
         return Scene(self, arrays)
 
+    def camera_rays(self, cam, pixels, samples, seed=0):
+        """First ray of each (pixel, sample) of a render with this camera:
+        (origins, unit directions) as (n, 3) float32.  Host buffers."""
+        p = np.ascontiguousarray(pixels, np.uint32)
+        s = np.ascontiguousarray(samples, np.uint32)
+        n = len(p)
+        o = np.zeros((max(n, 1), 3), np.float32)
+        d = np.zeros((max(n, 1), 3), np.float32)
+        c = cam.desc()
+        self.check(self.lib.aurora_camera_rays(self.h, C.byref(c), seed, p.ctypes.data, s.ctypes.data, n,
+                                               o.ctypes.data, d.ctypes.data))
+        return o[:n], d[:n]
+
+    def camera_rays_device(self, cam, pixels, samples, origins, dirs, seed=0):
+        """Device form: pixels/samples int32 or uint32 tensors (n,), origins and
+        dirs float32 (n, 3).  Asynchronous on the context stream."""
+        c = cam.desc()
+        self.check(self.lib.aurora_camera_rays_device(self.h, C.byref(c), seed, pixels.data_ptr(),
+                                                      samples.data_ptr(), pixels.shape[0], origins.data_ptr(),
+                                                      dirs.data_ptr()))
+
+    def resolve_rgb8_device(self, rgb_sum, spp, out, transfer=abi.TRANSFER_GAMMA22):
+        """8-bit resolve of a device radiance sum (H, W, 3) float32 into a
+        uint8 tensor of the same shape.  Asynchronous on the context stream."""
+        H, W = rgb_sum.shape[:2]
+        self.check(self.lib.aurora_resolve_rgb8_device(self.h, rgb_sum.data_ptr(), W, H, spp, transfer,
+                                                       out.data_ptr()))
+
 
 class Scene:
     def __init__(self, ctx: Context, arrays: SceneArrays):
@@ -328,12 +356,14 @@ def render_batch(ctx, items, graph=False):
     return stats_dict(st)
 
 
-def resolve_rgb8(rgb_sum, spp):
+def resolve_rgb8(rgb_sum, spp, transfer=abi.TRANSFER_GAMMA22):
+    """8-bit bytes (H, W, 3) of a radiance sum over spp samples, with the
+    transfer curve of experiments/rt.cpp:377-392 by default (abi.TRANSFER_*)."""
     lib = abi.load_library()
     a = np.ascontiguousarray(rgb_sum, np.float32)
     H, W = a.shape[:2]
     out = np.zeros((H, W, 3), np.uint8)
-    rc = lib.aurora_resolve_rgb8(a.ctypes.data, W, H, spp, out.ctypes.data)
+    rc = lib.aurora_resolve_rgb8(a.ctypes.data, W, H, spp, transfer, out.ctypes.data)
     if rc != abi.AURORA_OK:
         raise ValueError("aurora_resolve_rgb8: invalid arguments")
     return out

@@ -297,6 +297,30 @@ def stress_camera(width=1920, height=1080):
     return Camera(width, height, 45.0, (0.5, 0.5, -1.2), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 0.0, 1.7)
 
 
+def deep_chain_scene(levels=30, cluster=4096):
+    """A degenerate LBVH: a spine of `levels` binary levels, one per Morton
+    bit (from the top), each peeling off a group of three coincident spheres
+    whose box is larger than everything below, then `cluster` coincident tiny
+    spheres.  Centroids span exactly [0, 1]^3 (a tiny sphere at each corner),
+    so group k sits in the cell whose Morton code has only bit 29-k set.  The
+    greedy four-wide collapse opens each group before the spine, so it
+    advances one binary level per wide level and its walk's worst-case
+    stack need (three pushes per level) exceeds 96 entries."""
+    b = Builder()
+    m = b.lambertian((0.5, 0.5, 0.5))
+    b.sphere((1.0, 1.0, 1.0), 1e-4, m)
+    for k in range(levels):
+        axis, bit = k % 3, 9 - k // 3
+        c = [0.5 / 1024.0] * 3
+        c[axis] = (2.0 ** bit + 0.5) / 1024.0
+        r = 2.0 * 2.0 ** (-k / 3.0)
+        for _ in range(3):
+            b.sphere(tuple(c), r, m)
+    for _ in range(cluster):
+        b.sphere((0.0, 0.0, 0.0), 1e-4, m)
+    return b.arrays()
+
+
 def random_rays(n, lo, hi, seed=0):
     """n rays, origins uniform in [lo, hi]^3 and directions uniform on the
     unit sphere (Philox(i, 1, 0, 0) / (i, 2, 0, 0) with key (seed,

@@ -1105,16 +1105,69 @@ int aurora_render(aurora_scene* s, const aurora_camera_desc* cam, const aurora_r
     return AURORA_OK;
 }
 
-int aurora_resolve_rgb8(const float* rgb, uint32_t w, uint32_t h, uint32_t spp, uint8_t* out) {
-    if (!rgb || !out || spp == 0) return fail(nullptr, AURORA_ERR_INVALID, "NULL buffer or spp == 0");
-    const float scale = 1.0f / (float)spp;
-    const size_t n = (size_t)w * h * 3;
-    for (size_t i = 0; i < n; ++i) {
-        float x = rgb[i] * scale;
-        x = x > 0.0f ? sqrtf(x) : 0.0f;  // gamma 2
-        x = std::fmin(std::fmax(x, 0.0f), 0.999f);
-        out[i] = (uint8_t)(256.0f * x);
+int aurora_camera_rays_device(aurora_ctx* c, const aurora_camera_desc* cam, uint32_t seed, const uint32_t* pixels,
+                              const uint32_t* samples, uint32_t n, float* org, float* dir) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (!cam) return fail(c, AURORA_ERR_INVALID, "camera is NULL");
+    if (n && (!pixels || !samples || !org || !dir)) return fail(c, AURORA_ERR_INVALID, "NULL buffer");
+    CamF f;
+    const std::string e = camera_frame(*cam, f);
+    if (!e.empty()) return fail(c, AURORA_ERR_INVALID, e);
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    CK(c, launch_camera_rays(f, seed, pixels, samples, n, org, dir, c->stream));
+    return AURORA_OK;
+}
+
+int aurora_camera_rays(aurora_ctx* c, const aurora_camera_desc* cam, uint32_t seed, const uint32_t* pixels,
+                       const uint32_t* samples, uint32_t n, float* org, float* dir) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (!cam) return fail(c, AURORA_ERR_INVALID, "camera is NULL");
+    if (n == 0) return AURORA_OK;
+    if (!pixels || !samples || !org || !dir) return fail(c, AURORA_ERR_INVALID, "NULL buffer");
+    const uint64_t wh = (uint64_t)cam->image_width * cam->image_height;
+    for (uint32_t k = 0; k < n; ++k)
+        if (pixels[k] >= wh) return fail(c, AURORA_ERR_INVALID, "pixel index beyond width*height");
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    DevBuf b_pix, b_smp, b_o, b_d;
+    Status r = b_pix.alloc((size_t)n * 4);
+    if (!r.bad()) r = b_smp.alloc((size_t)n * 4);
+    if (!r.bad()) r = b_o.alloc((size_t)n * 12);
+    if (!r.bad()) r = b_d.alloc((size_t)n * 12);
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(b_pix.p, pixels, (size_t)n * 4, hipMemcpyHostToDevice, c->stream), "copy");
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(b_smp.p, samples, (size_t)n * 4, hipMemcpyHostToDevice, c->stream), "copy");
+    CamF f;
+    if (!r.bad()) {
+        const std::string e = camera_frame(*cam, f);
+        if (!e.empty()) r = Status{AURORA_ERR_INVALID, e};
     }
+    if (!r.bad()) r = launch_camera_rays(f, seed, b_pix.as<uint32_t>(), b_smp.as<uint32_t>(), n, b_o.as<float>(),
+                                         b_d.as<float>(), c->stream);
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(org, b_o.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(dir, b_d.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r.bad()) r = hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    (void)hipStreamSynchronize(c->stream);
+    if (r.bad()) return fail(c, r);
+    return AURORA_OK;
+}
+
+int aurora_resolve_rgb8(const float* rgb, uint32_t w, uint32_t h, uint32_t spp, uint32_t transfer, uint8_t* out) {
+    if (!rgb || !out || spp == 0) return fail(nullptr, AURORA_ERR_INVALID, "NULL buffer or spp == 0");
+    if (transfer > AURORA_TRANSFER_LINEAR) return fail(nullptr, AURORA_ERR_INVALID, "unknown transfer curve");
+    const float fs = (float)spp;
+    const size_t n = (size_t)w * h * 3;
+    for (size_t i = 0; i < n; ++i) out[i] = resolve_byte(rgb[i], fs, transfer);
+    return AURORA_OK;
+}
+
+int aurora_resolve_rgb8_device(aurora_ctx* c, const float* rgb, uint32_t w, uint32_t h, uint32_t spp,
+                               uint32_t transfer, uint8_t* out) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (!rgb || !out || spp == 0) return fail(c, AURORA_ERR_INVALID, "NULL buffer or spp == 0");
+    if (transfer > AURORA_TRANSFER_LINEAR) return fail(c, AURORA_ERR_INVALID, "unknown transfer curve");
+    if (((uintptr_t)rgb & 15) || ((uintptr_t)out & 3))
+        return fail(c, AURORA_ERR_INVALID, "rgb_sum must be 16-byte and out 4-byte aligned");
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    CK(c, launch_resolve_rgb8(rgb, (size_t)w * h * 3, spp, transfer, out, c->stream));
     return AURORA_OK;
 }
 

@@ -82,6 +82,34 @@ __host__ __device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 
 
 enum : uint32_t { STREAM_RENDER = 0u, STREAM_SCENE = 1u, STREAM_PERLIN = 2u };
 
+// ------------------------------------------------------- 8-bit resolve ---
+// One channel of a radiance sum over spp samples to a byte.  The mean is
+// sum / spp (one correctly rounded division); NaN counts as 0.
+//   GAMMA2  : "Ray Tracing in One Weekend" write_color: sqrt (0 for x <= 0),
+//             clamped to [0, 0.999], times 256, truncated.
+//   GAMMA22 : the writer of experiments/rt.cpp:377-392: clamp to [0, 1],
+//             pow(c, 1/2.2f) times 255 in float, truncated.  The power is
+//             evaluated in double and rounded once to float (the correctly
+//             rounded powf except within 2^-29 of a float midpoint), so host
+//             and device agree.
+//   LINEAR  : are::Texture::save_texture (src/texture.cpp:378-391):
+//             clamp(c * 255.0, 0, 255) in double, truncated.
+__host__ __device__ __forceinline__ uint8_t resolve_byte(float sum, float spp, uint32_t transfer) {
+    float x = sum / spp;
+    if (!(x == x)) x = 0.0f;
+    if (transfer == AURORA_TRANSFER_GAMMA22) {
+        const float c = fminf(fmaxf(x, 0.0f), 1.0f);
+        const float g = (float)pow((double)c, (double)(1.0f / 2.2f));
+        return (uint8_t)(g * 255.0f);
+    }
+    if (transfer == AURORA_TRANSFER_LINEAR) {
+        const double y = (double)x * 255.0;
+        return (uint8_t)(y < 0.0 ? 0.0 : y > 255.0 ? 255.0 : y);
+    }
+    const float g = x > 0.0f ? sqrtf(x) : 0.0f;
+    return (uint8_t)(256.0f * fminf(fmaxf(g, 0.0f), 0.999f));
+}
+
 // ------------------------------------------------ sin / atan2 / acos (f64) ---
 // Written from the series definitions (reduction by pi/2 with a two-part
 // constant, Taylor polynomials; two half-angle steps then the atan series),

@@ -180,6 +180,12 @@ struct Workspace {
     uint32_t* host_counts = nullptr;  // pinned, 256 slots
 };
 Status launch_raygen(const Pass& p, Workspace& ws, hipStream_t s);
+// camera rays of n (pixel, sample) pairs into 3*n float origins/directions
+// (pixel = j*W + i < W*H, checked by the caller)
+Status launch_camera_rays(const CamF& cam, uint32_t seed, const uint32_t* pixels, const uint32_t* samples,
+                          uint32_t n, float* org, float* dir, hipStream_t s);
+// 8-bit resolve of n floats (sum and out 16-byte and 4-byte aligned)
+Status launch_resolve_rgb8(const float* sum, size_t n, uint32_t spp, uint32_t transfer, uint8_t* out, hipStream_t s);
 Status launch_shade(const Pass& p, Workspace& ws, int cur, uint32_t depth, int grid, hipStream_t s);
 Status launch_tail(const Pass& p, Workspace& ws, int cur, uint32_t depth, int grid, hipStream_t s);
 Status launch_accumulate(const Pass& p, Workspace& ws, hipStream_t s);

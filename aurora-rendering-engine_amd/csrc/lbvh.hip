@@ -23,6 +23,7 @@
 // No reference counterpart: the reference scans every triangle for every ray
 // (experiments/rt.cpp:209-218).
 #include "internal.hpp"
+#include "traverse.hpp"
 
 namespace aur {
 
@@ -447,7 +448,7 @@ __global__ __launch_bounds__(kBlock) void k_jump_out(uint32_t ni, const uint2* J
                                                      uint32_t* wflag) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i > ni) return;
-    wflag[i] = i == 0 && ni > 0 ? 1u : 0u;
+    wflag[i] = i == 0 && ni > 0 ? 1u : 0u;  // wide level of the root
     if (i == ni) return;
     const uint2 a = J[i];
     const uint32_t d = (a.x == kNone && a.y < 255u) ? a.y : 255u;
@@ -561,18 +562,40 @@ __device__ __forceinline__ int wide_cut(uint32_t i, const uint32_t* gamma, const
     return k;
 }
 
-// the flagged internal nodes of one depth flag the internal nodes of their cuts
+// the flagged internal nodes of one depth flag the internal nodes of their
+// cuts with their own wide level plus one (the root's level is 1)
 __global__ __launch_bounds__(kBlock) void k_wide_mark(const uint32_t* lvl_node, uint32_t count,
                                                       const uint32_t* gamma, const uint32_t* flags,
                                                       const float4* nodes, bool greedy, uint32_t* wflag) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= count) return;
     const uint32_t i = lvl_node[k];
-    if (!wflag[i]) return;
+    const uint32_t lv = wflag[i];
+    if (!lv) return;
     WideRef r[4];
     const int m = wide_cut(i, gamma, flags, nodes, greedy, r);
     for (int q = 0; q < m; ++q)
-        if (!r[q].leaf) wflag[r[q].j] = 1u;
+        if (!r[q].leaf) wflag[r[q].j] = lv + 1u;
+}
+
+// the deepest wide level into *maxlv (0 before the launch), and the levels
+// turned into the 0/1 flags that the scan numbers
+__global__ __launch_bounds__(kBlock) void k_wide_levels(uint32_t n, uint32_t* wflag, uint32_t* maxlv) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t lv = 0u;
+    if (i < n) {
+        lv = wflag[i];
+        wflag[i] = lv ? 1u : 0u;
+    }
+    for (int o = 32; o > 0; o >>= 1) lv = max(lv, (uint32_t)__shfl_xor((int)lv, o));
+    if ((threadIdx.x & 63) == 0 && lv) atomicMax(maxlv, lv);
+}
+
+// back to the root's flag alone (a second collapse, pairwise)
+__global__ __launch_bounds__(kBlock) void k_wide_reset(uint32_t n, uint32_t* wflag, uint32_t* maxlv) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) wflag[i] = i == 0 ? 1u : 0u;
+    if (i == 0) *maxlv = 0u;
 }
 
 __device__ __forceinline__ void wide_slot(float4* w, int k, const float4* nodes, uint32_t pos, uint32_t ref) {
@@ -706,7 +729,8 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
                                          s.vals[b].as<uint32_t>(), n, 8 * pass, s.hist.as<uint32_t>(), nb);
     }
     // four passes: sorted data is back in buffer 0
-    uint32_t depth = 0, n_wide = 1;
+    uint32_t depth = 0, n_wide = 1, wide_depth = 1;
+    bool wide_greedy = false;
     if (n > 1) {
         const uint32_t nint = n - 1;
         const uint32_t gi = blocks_for(nint, kBlock);
@@ -743,16 +767,33 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
                 s.lvl_node.as<uint32_t>() + starts[d], cnt, s.gamma.as<uint32_t>(), s.int_pos.as<uint32_t>(),
                 s.int_parent.as<uint32_t>(), nodes.as<float4>());
         }
-        // four-wide traversal tree: flags top down, then their scan and the fill
-        const bool greedy = n >= kWideGreedyMin;
-        for (uint32_t d = 0; d <= maxd; ++d) {
-            const uint32_t cnt = starts[d + 1] - starts[d];
-            if (cnt == 0) continue;
-            k_wide_mark<<<blocks_for(cnt, kBlock), kBlock, 0, st>>>(s.lvl_node.as<uint32_t>() + starts[d], cnt,
-                                                                     s.gamma.as<uint32_t>(), s.flags.as<uint32_t>(),
-                                                                     nodes.as<float4>(), greedy,
-                                                                     s.widx.as<uint32_t>());
+        // four-wide traversal tree: levels top down, then the flags' scan and
+        // the fill.  The walk pushes at most three nodes per wide level above
+        // the deepest, so the greedy form is kept only while that fits the
+        // traversal stack; the pairwise form has ceil(depth / 2) levels, which
+        // fits every tree of depth <= 64
+        if (depth > (uint32_t)kTravMaxStack)
+            return Status{AURORA_ERR_RUNTIME, "LBVH build: tree deeper than the traversal stack"};
+        bool greedy = n >= kWideGreedyMin;
+        uint32_t* maxlv = s.bounds.as<uint32_t>() + 12;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            k_wide_reset<<<blocks_for(nint + 1, kBlock), kBlock, 0, st>>>(nint + 1, s.widx.as<uint32_t>(), maxlv);
+            for (uint32_t d = 0; d <= maxd; ++d) {
+                const uint32_t cnt = starts[d + 1] - starts[d];
+                if (cnt == 0) continue;
+                k_wide_mark<<<blocks_for(cnt, kBlock), kBlock, 0, st>>>(s.lvl_node.as<uint32_t>() + starts[d], cnt,
+                                                                         s.gamma.as<uint32_t>(),
+                                                                         s.flags.as<uint32_t>(), nodes.as<float4>(),
+                                                                         greedy, s.widx.as<uint32_t>());
+            }
+            k_wide_levels<<<blocks_for(nint + 1, kBlock), kBlock, 0, st>>>(nint + 1, s.widx.as<uint32_t>(), maxlv);
+            AUR_HIP(hipMemcpyAsync(&wide_depth, maxlv, 4, hipMemcpyDeviceToHost, st));
+            AUR_HIP(hipStreamSynchronize(st));
+            if (3u * (wide_depth - 1u) <= (uint32_t)kTravMaxStack) break;
+            if (!greedy) return Status{AURORA_ERR_RUNTIME, "LBVH build: four-wide tree deeper than the traversal stack"};
+            greedy = false;
         }
+        wide_greedy = greedy;
         scan_exclusive(s.widx.as<uint32_t>(), nint + 1, st);
         k_wide_fill<<<gi, kBlock, 0, st>>>(nint, s.gamma.as<uint32_t>(), s.flags.as<uint32_t>(),
                                            s.widx.as<uint32_t>(), nodes.as<float4>(), greedy, wide.as<float4>());
@@ -776,6 +817,8 @@ Status lbvh_build(const DevScene& sc_in, uint32_t n, DevBuf& nodes, DevBuf& wide
     local.n_nodes = n_nodes;
     local.n_wide = n_wide;
     local.depth = depth;
+    local.wide_depth = wide_depth;
+    local.wide_form = wide_greedy ? AURORA_BVH4_GREEDY : AURORA_BVH4_PAIRWISE;
     for (int k = 0; k < 3; ++k) {
         local.scene_lo[k] = o2f(b[k]);
         local.scene_hi[k] = o2f(b[3 + k]);

@@ -39,6 +39,34 @@ __device__ __forceinline__ uint32_t find_rec(const SceneRec* recs, uint32_t n, u
     return lo;
 }
 
+// Camera ray of one sample through pixel (i, j) (pixel = j*W + i): the pixel
+// centre p00 + i*du + j*dv jittered by U[-1/2, 1/2)^2 from Philox(pixel,
+// sample, 0, 0), and, with depth of field, the origin moved to a point of the
+// lens disk found by rejection among up to 32 draws (pixel, sample, 0, 1+k).
+// The role of experiments/rt.cpp:364-366 (pixel -> direction), with the
+// "Ray Tracing in One Weekend" viewport the host computes in camera_frame.
+__device__ __forceinline__ void camera_ray(const CamF& cam, uint32_t pixel, uint32_t i, uint32_t j,
+                                           uint32_t sample, uint32_t seed, v3& org, v3& dir) {
+    const u4 q = philox(u4{pixel, sample, 0u, 0u}, seed, STREAM_RENDER);
+    const float ox = u01(q.x) - 0.5f, oy = u01(q.y) - 0.5f;
+    const v3 ps = (cam.p00 + ((float)i + ox) * cam.du) + ((float)j + oy) * cam.dv;
+    org = cam.center;
+    if (cam.defocus) {
+        float px = 0.0f, py = 0.0f;
+        for (uint32_t k = 0; k < 32; ++k) {
+            const u4 c = philox(u4{pixel, sample, 0u, 1u + k}, seed, STREAM_RENDER);
+            const float ax = 2.0f * u01(c.x) - 1.0f, ay = 2.0f * u01(c.y) - 1.0f;
+            if (ax * ax + ay * ay < 1.0f) {
+                px = ax;
+                py = ay;
+                break;
+            }
+        }
+        org = (org + px * cam.disk_u) + py * cam.disk_v;
+    }
+    dir = unitv(ps - org);
+}
+
 // ONE (in every kernel below): the pass holds a single scene, so its record
 // is read at a wave-uniform address (scalar loads) instead of per path.
 template <bool ONE>
@@ -58,29 +86,30 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Pass pa, float4* qo, float4* 
     const uint32_t s_local = local / r.WH;
     const uint32_t pixel = local - s_local * r.WH;
     const uint32_t i = pixel % r.cam.W, j = pixel / r.cam.W;
-    const uint32_t sample = r.sample_base + s_local;
-    const u4 q = philox(u4{pixel, sample, 0u, 0u}, r.seed, STREAM_RENDER);
-    const float ox = u01(q.x) - 0.5f, oy = u01(q.y) - 0.5f;
-    const v3 ps = (r.cam.p00 + ((float)i + ox) * r.cam.du) + ((float)j + oy) * r.cam.dv;
-    v3 org = r.cam.center;
-    if (r.cam.defocus) {
-        float px = 0.0f, py = 0.0f;
-        for (uint32_t k = 0; k < 32; ++k) {
-            const u4 c = philox(u4{pixel, sample, 0u, 1u + k}, r.seed, STREAM_RENDER);
-            const float ax = 2.0f * u01(c.x) - 1.0f, ay = 2.0f * u01(c.y) - 1.0f;
-            if (ax * ax + ay * ay < 1.0f) {
-                px = ax;
-                py = ay;
-                break;
-            }
-        }
-        org = (org + px * r.cam.disk_u) + py * r.cam.disk_v;
-    }
-    const v3 dir = unitv(ps - org);
+    v3 org, dir;
+    camera_ray(r.cam, pixel, i, j, r.sample_base + s_local, r.seed, org, dir);
     qo[p] = make_float4(org.x, org.y, org.z, __uint_as_float(p));
     qd[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(sid));
     qb[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Camera rays of n (pixel, sample) pairs (the parity entry
+// aurora_camera_rays_device): the same camera_ray as k_raygen.
+__global__ __launch_bounds__(kBlock) void k_camera_rays(CamF cam, uint32_t seed, const uint32_t* pixels,
+                                                        const uint32_t* samples, uint32_t n, float* org,
+                                                        float* dir) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t pixel = pixels[k];
+    v3 o, d;
+    camera_ray(cam, pixel, pixel % cam.W, pixel / cam.W, samples[k], seed, o, d);
+    org[3 * (size_t)k] = o.x;
+    org[3 * (size_t)k + 1] = o.y;
+    org[3 * (size_t)k + 2] = o.z;
+    dir[3 * (size_t)k] = d.x;
+    dir[3 * (size_t)k + 1] = d.y;
+    dir[3 * (size_t)k + 2] = d.z;
 }
 
 // sin and cos of 2*pi*u for u in [0, 1) (u a multiple of 2^-24): exact
@@ -479,6 +508,37 @@ Status launch_raygen(const Pass& p, Workspace& ws, hipStream_t s) {
     auto k = p.n_recs == 1 ? k_raygen<true> : k_raygen<false>;
     k<<<blocks, kBlock, 0, s>>>(p, ws.q_o[0].as<float4>(), ws.q_d[0].as<float4>(), ws.q_b[0].as<float4>(),
                                 ws.L.as<float4>(), ws.counters.as<uint32_t>());
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status launch_camera_rays(const CamF& cam, uint32_t seed, const uint32_t* pixels, const uint32_t* samples,
+                          uint32_t n, float* org, float* dir, hipStream_t s) {
+    if (n == 0) return Status::ok();
+    k_camera_rays<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(cam, seed, pixels, samples, n, org, dir);
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+// 8-bit resolve of n floats (three per pixel), four per thread
+__global__ __launch_bounds__(kBlock) void k_resolve_rgb8(const float* sum, size_t n, float spp, uint32_t transfer,
+                                                         uint8_t* out) {
+    const size_t k = ((size_t)blockIdx.x * kBlock + threadIdx.x) * 4;
+    if (k + 4 <= n) {
+        const float4 v = *reinterpret_cast<const float4*>(sum + k);
+        const uint32_t b = (uint32_t)resolve_byte(v.x, spp, transfer) | ((uint32_t)resolve_byte(v.y, spp, transfer) << 8) |
+                           ((uint32_t)resolve_byte(v.z, spp, transfer) << 16) |
+                           ((uint32_t)resolve_byte(v.w, spp, transfer) << 24);
+        *reinterpret_cast<uint32_t*>(out + k) = b;
+    } else {
+        for (size_t i = k; i < n; ++i) out[i] = resolve_byte(sum[i], spp, transfer);
+    }
+}
+
+Status launch_resolve_rgb8(const float* sum, size_t n, uint32_t spp, uint32_t transfer, uint8_t* out, hipStream_t s) {
+    if (n == 0) return Status::ok();
+    const size_t blocks = (n + 4 * kBlock - 1) / (4 * kBlock);
+    k_resolve_rgb8<<<(uint32_t)blocks, kBlock, 0, s>>>(sum, n, (float)spp, transfer, out);
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }

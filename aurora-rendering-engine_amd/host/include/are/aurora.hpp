@@ -303,8 +303,12 @@ class camera {
     double defocus_angle = 0;
     double focus_dist = 10;
     uint32_t seed = 0;
+    // transfer curve of render_ppm (AURORA_TRANSFER_*): by default the
+    // writer of experiments/rt.cpp:377-392 (pow(c, 1/2.2f) x 255)
+    uint32_t transfer = AURORA_TRANSFER_GAMMA22;
 
-    // Render and write the image to std::cout as ASCII PPM (P3), gamma 2.
+    // Render and write the image to std::cout as ASCII PPM (P3), gamma 2
+    // ("Ray Tracing in One Weekend" write_color).
     void render(const hittable& world);
     // Render into `out`: linear RGB means, row 0 = top (height x width x 3).
     void render(const hittable& world, std::vector<float>& out);
@@ -316,6 +320,12 @@ class camera {
    private:
     void render_sum(const hittable& world, std::vector<float>& sum);
 };
+
+// Write 8-bit RGB bytes (width*height*3, rows top-down) as a binary P6 PPM,
+// the layout of are::Texture::save_texture (src/texture.cpp:372-391):
+// header "P6\n<w> <h>\n255\n", then the bytes.  Throws std::runtime_error
+// when the file cannot be written.
+void write_ppm(const std::string& path, int width, int height, const std::vector<uint8_t>& rgb8);
 
 // Process-wide libaurora context on `device` (created on first use).
 aurora_ctx* default_context(int device = 0);

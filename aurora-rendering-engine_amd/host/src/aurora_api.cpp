@@ -330,7 +330,7 @@ void camera::render(const hittable& world) {
     const int h = image_height();
     std::vector<uint8_t> px(sum.size());
     check(nullptr, aurora_resolve_rgb8(sum.data(), (uint32_t)image_width, (uint32_t)h, (uint32_t)samples_per_pixel,
-                                       px.data()));
+                                       AURORA_TRANSFER_GAMMA2, px.data()));
     std::cout << "P3\n" << image_width << ' ' << h << "\n255\n";
     for (size_t i = 0; i < px.size(); i += 3) std::cout << (int)px[i] << ' ' << (int)px[i + 1] << ' ' << (int)px[i + 2] << '\n';
 }
@@ -341,12 +341,18 @@ void camera::render_ppm(const hittable& world, const std::string& path) {
     const int h = image_height();
     std::vector<uint8_t> px(sum.size());
     check(nullptr, aurora_resolve_rgb8(sum.data(), (uint32_t)image_width, (uint32_t)h, (uint32_t)samples_per_pixel,
-                                       px.data()));
+                                       transfer, px.data()));
+    write_ppm(path, image_width, h, px);
+}
+
+void write_ppm(const std::string& path, int width, int height, const std::vector<uint8_t>& rgb8) {
+    if (width <= 0 || height <= 0 || rgb8.size() != (size_t)width * height * 3)
+        throw std::invalid_argument("image size does not match width*height*3");
     FILE* fp = fopen(path.c_str(), "wb");
-    if (!fp) throw std::runtime_error("cannot open " + path);
-    fprintf(fp, "P6\n%d %d\n255\n", image_width, h);
-    fwrite(px.data(), 1, px.size(), fp);
-    fclose(fp);
+    if (!fp) throw std::runtime_error("Failed to open file for writing: " + path);
+    const bool ok = fprintf(fp, "P6\n%d %d\n255\n", width, height) > 0 &&
+                    fwrite(rgb8.data(), 1, rgb8.size(), fp) == rgb8.size();
+    if (fclose(fp) != 0 || !ok) throw std::runtime_error("Failed to write file: " + path);
 }
 
 }  // namespace are

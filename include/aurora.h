@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define AURORA_ABI_VERSION 2
+#define AURORA_ABI_VERSION 3
 
 /* ---- status codes ---------------------------------------------------- */
 #define AURORA_OK 0
@@ -143,11 +143,18 @@ typedef struct aurora_bvh4_node {
     aurora_bvh4_slot slot[4];
 } aurora_bvh4_node;
 
+#define AURORA_BVH4_PAIRWISE 0u
+#define AURORA_BVH4_GREEDY 1u
 typedef struct aurora_bvh_stats {
     uint32_t n_prims;
     uint32_t n_nodes;       /* 2*n_prims (1 for a single primitive)         */
     uint32_t depth;         /* longest root-to-leaf path, in edges          */
     uint32_t n_wide;        /* four-wide traversal nodes                    */
+    uint32_t wide_depth;    /* four-wide nodes on the longest root path     */
+    uint32_t wide_form;     /* AURORA_BVH4_GREEDY from 4096 primitives while
+                               its walk fits the 96-entry traversal stack
+                               (three pushes per level above the deepest),
+                               else AURORA_BVH4_PAIRWISE                    */
     float scene_lo[3], scene_hi[3];     /* union of primitive boxes         */
     float centroid_lo[3], centroid_hi[3];/* Morton quantisation frame        */
     double build_ms;        /* device time of the build kernels            */
@@ -338,11 +345,48 @@ int aurora_render_batch_device(aurora_ctx* ctx, const aurora_batch_item* items,
                                uint32_t n, uint32_t flags,
                                aurora_render_stats* stats);
 
-/* Resolve a radiance sum to 8-bit sRGB-ish bytes: x = clamp(sqrt(sum/spp),
- * 0, 0.999) * 256, truncated; byte layout of a P6 PPM body
- * (src/texture.cpp:378-391).  Host buffers. */
+/* Camera rays of n (pixel, sample) pairs: ray k is the first ray that
+ * aurora_render traces for sample samples[k] of pixel pixels[k] (pixel =
+ * row * image_width + column, row 0 at the top) under Philox key word
+ * `seed` (aurora_render_desc.seed): the pixel centre jittered within its
+ * pixel, from the centre of projection or, with defocus_angle > 0, from a
+ * point of the lens disk of radius focus_dist * tan(defocus_angle / 2).
+ * The role of experiments/rt.cpp:337-343 (camera basis) and :364-366
+ * (pixel -> direction), on the "Ray Tracing in One Weekend" viewport
+ * (vertical field of view vfov, aspect width/height).  origins and dirs
+ * receive 3*n floats (unit directions).  Host buffers; pixels must be below
+ * width*height. */
+int aurora_camera_rays(aurora_ctx* ctx, const aurora_camera_desc* cam,
+                       uint32_t seed, const uint32_t* pixels,
+                       const uint32_t* samples, uint32_t n, float* origins,
+                       float* dirs);
+/* Same on device buffers (a pixel beyond width*height gives the ray through
+ * that position past the image's last row). */
+int aurora_camera_rays_device(aurora_ctx* ctx, const aurora_camera_desc* cam,
+                              uint32_t seed, const uint32_t* d_pixels,
+                              const uint32_t* d_samples, uint32_t n,
+                              float* d_origins, float* d_dirs);
+
+/* Transfer curves of the 8-bit resolve.  The mean is sum / spp; NaN
+ * counts as 0; each result is truncated to a byte. */
+#define AURORA_TRANSFER_GAMMA2 0u   /* "Ray Tracing in One Weekend":
+                                       256 * clamp(sqrt(x), 0, 0.999)       */
+#define AURORA_TRANSFER_GAMMA22 1u  /* experiments/rt.cpp:377-392:
+                                       255 * pow(clamp(x, 0, 1), 1/2.2f)    */
+#define AURORA_TRANSFER_LINEAR 2u   /* are::Texture::save_texture
+                                       (src/texture.cpp:378-391):
+                                       clamp(255 * x, 0, 255)               */
+
+/* Resolve a radiance sum (W*H*3 floats, rows top-down) to bytes in the
+ * layout of a P6 PPM body (src/texture.cpp:378-391; header
+ * "P6\n<w> <h>\n255\n" as at src/texture.cpp:375).  Host buffers. */
 int aurora_resolve_rgb8(const float* rgb_sum, uint32_t width,
-                        uint32_t height, uint32_t spp, uint8_t* out);
+                        uint32_t height, uint32_t spp, uint32_t transfer,
+                        uint8_t* out);
+/* Same on device buffers, on the context's stream (asynchronous). */
+int aurora_resolve_rgb8_device(aurora_ctx* ctx, const float* d_rgb_sum,
+                               uint32_t width, uint32_t height, uint32_t spp,
+                               uint32_t transfer, uint8_t* d_out);
 
 /* ---- learned denoiser ---------------------------------------------------
  * A 4-level U-Net (32/48/64/64 channels, 15 3x3 convolutions) on BF16 MFMA

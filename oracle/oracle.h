@@ -62,6 +62,12 @@ void orc_texture_value(orc_scene* s, uint32_t tex, float u, float v, const float
 void orc_sphere_uv(const float* n, float* uv);
 /* out18: center, pixel00, delta_u, delta_v, disk_u, disk_v */
 int orc_camera(const aurora_camera_desc* c, float* out18);
+/* first ray of sample samples[k] through pixel pixels[k] (row-major), as
+ * aurora_camera_rays: 3*n origins and unit directions */
+int orc_camera_rays(const aurora_camera_desc* c, uint32_t seed, const uint32_t* pixels, const uint32_t* samples,
+                    uint32_t n, float* org, float* dir);
+/* 8-bit resolve of n floats (sum over spp samples), AURORA_TRANSFER_* */
+int orc_resolve_rgb8(const float* sum, uint64_t n, uint32_t spp, uint32_t transfer, uint8_t* out);
 /* pixels NULL = every pixel in row-major order; rgb/albedo/normal are 3 per
  * pixel and are added to (sum over samples), as libaurora does */
 int orc_render_pixels(orc_scene* s, const aurora_camera_desc* c, const aurora_render_desc* rd,

@@ -52,6 +52,8 @@ def lib():
             "orc_texture_value": (None, [vp, u32, f, f, vp, vp]),
             "orc_sphere_uv": (None, [vp, vp]),
             "orc_camera": (C.c_int, [C.POINTER(abi.CameraDesc), vp]),
+            "orc_camera_rays": (C.c_int, [C.POINTER(abi.CameraDesc), u32, vp, vp, u32, vp, vp]),
+            "orc_resolve_rgb8": (C.c_int, [vp, C.c_uint64, u32, u32, vp]),
             "orc_render_pixels": (C.c_int, [vp, C.POINTER(abi.CameraDesc), C.POINTER(abi.RenderDesc), vp, u32,
                                             C.c_int, vp, vp, vp, vp]),
             "orc_render_pixels_f64": (C.c_int, [vp, C.POINTER(abi.CameraDesc), C.POINTER(abi.RenderDesc), vp,
@@ -141,6 +143,29 @@ def camera_frame(cam):
     return out.reshape(6, 3)
 
 
+def camera_rays(cam, pixels, samples, seed=0):
+    """First ray of each (pixel, sample): (origins, unit directions), n x 3."""
+    p = np.ascontiguousarray(pixels, np.uint32)
+    s = np.ascontiguousarray(samples, np.uint32)
+    n = len(p)
+    o = np.zeros((n, 3), np.float32)
+    d = np.zeros((n, 3), np.float32)
+    c = cam.desc()
+    rc = lib().orc_camera_rays(C.byref(c), seed, p.ctypes.data, s.ctypes.data, n, o.ctypes.data, d.ctypes.data)
+    if rc:
+        raise ValueError(_err())
+    return o, d
+
+
+def resolve_rgb8(rgb_sum, spp, transfer):
+    a = np.ascontiguousarray(rgb_sum, np.float32)
+    out = np.zeros(a.shape, np.uint8)
+    rc = lib().orc_resolve_rgb8(a.ctypes.data, a.size, spp, transfer, out.ctypes.data)
+    if rc:
+        raise ValueError(_err())
+    return out
+
+
 def hit_sphere(sphere, o, d, tmin=0.0, tmax=np.inf):
     s = np.asarray(sphere, np.float32)
     oo = np.asarray(o, np.float32)
@@ -179,7 +204,8 @@ class OracleScene:
 
     def build_bvh(self):
         st = abi.BvhStats()
-        lib().orc_bvh_build(self.h, C.byref(st))
+        if lib().orc_bvh_build(self.h, C.byref(st)):
+            raise RuntimeError(_err())
         from aurora_amd.core import stats_dict
         self.stats = stats_dict(st)
         return self.stats

@@ -12,7 +12,9 @@
 //     reference's estimator is AO + cosine bounces + Russian roulette; this
 //     restates the hit -> emitted -> scatter -> attenuate loop instead).
 //   reflect/refract: src/basic/vec3.cpp:182-199.
+#include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -147,19 +149,19 @@ struct PathOut {
     uint32_t segments;
 };
 
-// One camera sample through pixel (i, j): the wavefront loop of libaurora,
-// restated as a single sequential loop.
+// First ray of one sample through pixel (i, j): the pixel centre
+// p00 + i*du + j*dv jittered by U[-1/2, 1/2)^2 from Philox(pixel, sample, 0,
+// 0); with depth of field the origin is a lens-disk point found by rejection
+// among up to 32 draws (pixel, sample, 0, 1+k).  Follows the pixel mapping of
+// experiments/rt.cpp:364-366 on the viewport of camera_setup.
 template <class R>
-static V3<R> trace_path(const Scene& s, const Cam& cam, const aurora_render_desc& rd, uint32_t i, uint32_t j,
-                        uint32_t sample, V3<R>* alb, V3<R>* nrm, uint32_t& segments) {
-    const uint32_t pixel = j * cam.W + i;
-    const uint32_t seed = rd.seed;
-    const R tmin = rd.tmin > 0.0f ? R(rd.tmin) : R(0.001f);
+static void camera_ray(const Cam& cam, uint32_t pixel, uint32_t i, uint32_t j, uint32_t sample, uint32_t seed,
+                       V3<R>& org, V3<R>& dir) {
     U4 r = philox4x32_10(U4{pixel, sample, 0u, 0u}, seed, STREAM_RENDER);
     R ox = R(u01f(r.x)) - R(0.5), oy = R(u01f(r.y)) - R(0.5);
     V3<R> p00 = cvt<R>(cam.p00), du = cvt<R>(cam.du), dv = cvt<R>(cam.dv);
     V3<R> ps = (p00 + (R(float(i)) + ox) * du) + (R(float(j)) + oy) * dv;
-    V3<R> org = cvt<R>(cam.center);
+    org = cvt<R>(cam.center);
     if (cam.defocus) {
         R px = R(0), py = R(0);
         for (uint32_t k = 0; k < 32; ++k) {
@@ -169,7 +171,19 @@ static V3<R> trace_path(const Scene& s, const Cam& cam, const aurora_render_desc
         }
         org = (org + px * cvt<R>(cam.disk_u)) + py * cvt<R>(cam.disk_v);
     }
-    V3<R> dir = unit(ps - org);
+    dir = unit(ps - org);
+}
+
+// One camera sample through pixel (i, j): the wavefront loop of libaurora,
+// restated as a single sequential loop.
+template <class R>
+static V3<R> trace_path(const Scene& s, const Cam& cam, const aurora_render_desc& rd, uint32_t i, uint32_t j,
+                        uint32_t sample, V3<R>* alb, V3<R>* nrm, uint32_t& segments) {
+    const uint32_t pixel = j * cam.W + i;
+    const uint32_t seed = rd.seed;
+    const R tmin = rd.tmin > 0.0f ? R(rd.tmin) : R(0.001f);
+    V3<R> org, dir;
+    camera_ray<R>(cam, pixel, i, j, sample, seed, org, dir);
     V3<R> L = mk<R>(0, 0, 0), beta = mk<R>(1, 1, 1);
     const V3<R> bg = mk<R>(rd.background[0], rd.background[1], rd.background[2]);
     const uint32_t ns = uint32_t(s.spheres.size());
@@ -392,7 +406,13 @@ int orc_scene_create(const aurora_scene_desc* d, orc_scene** out) {
 void orc_scene_destroy(orc_scene* s) { delete s; }
 
 int orc_bvh_build(orc_scene* sc, aurora_bvh_stats* st) {
-    build_lbvh(sc->s);
+    try {
+        build_lbvh(sc->s);
+    } catch (const std::exception& e) {
+        sc->s.built = false;
+        g_err = e.what();
+        return AURORA_ERR_RUNTIME;
+    }
     if (st) *st = sc->s.stats;
     return AURORA_OK;
 }
@@ -498,6 +518,46 @@ int orc_camera(const aurora_camera_desc* c, float* out18) {
     if (!e.empty()) { g_err = e; return AURORA_ERR_INVALID; }
     V3<float> v[6] = {cam.center, cam.p00, cam.du, cam.dv, cam.disk_u, cam.disk_v};
     for (int i = 0; i < 6; ++i) { out18[3 * i] = v[i].x; out18[3 * i + 1] = v[i].y; out18[3 * i + 2] = v[i].z; }
+    return AURORA_OK;
+}
+
+int orc_camera_rays(const aurora_camera_desc* c, uint32_t seed, const uint32_t* pixels, const uint32_t* samples,
+                    uint32_t n, float* org, float* dir) {
+    Cam cam;
+    std::string e = camera_setup(*c, cam);
+    if (!e.empty()) { g_err = e; return AURORA_ERR_INVALID; }
+    for (uint32_t k = 0; k < n; ++k) {
+        V3<float> o, d;
+        camera_ray<float>(cam, pixels[k], pixels[k] % cam.W, pixels[k] / cam.W, samples[k], seed, o, d);
+        org[3 * k] = o.x; org[3 * k + 1] = o.y; org[3 * k + 2] = o.z;
+        dir[3 * k] = d.x; dir[3 * k + 1] = d.y; dir[3 * k + 2] = d.z;
+    }
+    return AURORA_OK;
+}
+
+// 8-bit resolve of one channel (transfer curves as include/aurora.h states):
+// GAMMA2 after "Ray Tracing in One Weekend" write_color, GAMMA22 after the
+// writer of experiments/rt.cpp:377-392 (clamp, pow(c, 1/2.2f), x255),
+// LINEAR after are::Texture::save_texture (src/texture.cpp:378-391).
+static uint8_t resolve_byte(float sum, float spp, uint32_t transfer) {
+    float x = sum / spp;
+    if (std::isnan(x)) x = 0.0f;
+    if (transfer == AURORA_TRANSFER_GAMMA22) {
+        const float c = std::min(std::max(x, 0.0f), 1.0f);
+        const float g = float(std::pow(double(c), double(1.0f / 2.2f)));
+        return uint8_t(g * 255.0f);
+    }
+    if (transfer == AURORA_TRANSFER_LINEAR) {
+        const double y = double(x) * 255.0;
+        return uint8_t(std::clamp(y, 0.0, 255.0));
+    }
+    const float g = x > 0.0f ? std::sqrt(x) : 0.0f;
+    return uint8_t(256.0f * std::min(std::max(g, 0.0f), 0.999f));
+}
+
+int orc_resolve_rgb8(const float* sum, uint64_t n, uint32_t spp, uint32_t transfer, uint8_t* out) {
+    if (spp == 0 || transfer > AURORA_TRANSFER_LINEAR) { g_err = "spp == 0 or unknown transfer"; return AURORA_ERR_INVALID; }
+    for (uint64_t i = 0; i < n; ++i) out[i] = resolve_byte(sum[i], float(spp), transfer);
     return AURORA_OK;
 }
 

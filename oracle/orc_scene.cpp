@@ -214,6 +214,7 @@ static uint32_t quantise(float c, float lo, float hi) {
 // the same tree: aurora lbvh.hip wide_cut, k_wide_mark, k_wide_fill.)  No
 // reference counterpart.
 static constexpr uint32_t kWideGreedyMin = 4096;
+static constexpr uint32_t kStack = 96;  // entries of the traversal stack
 
 static void collapse_wide(Scene& s, const uint32_t* gamma, const uint8_t* lleaf, const uint8_t* rleaf,
                           uint32_t ni) {
@@ -244,7 +245,7 @@ static void collapse_wide(Scene& s, const uint32_t* gamma, const uint8_t* lleaf,
         const float dx = nd.hi[0] - nd.lo[0], dy = nd.hi[1] - nd.lo[1], dz = nd.hi[2] - nd.lo[2];
         return dx * dy + dy * dz + dz * dx;
     };
-    const bool greedy = n >= kWideGreedyMin;
+    bool greedy = n >= kWideGreedyMin;
     // children of internal i: positions 2+2g (internal index g) and 3+2g (g+1)
     auto cut = [&](uint32_t i, Ref* out) {
         const uint32_t g = gamma[i];
@@ -282,17 +283,38 @@ static void collapse_wide(Scene& s, const uint32_t* gamma, const uint8_t* lleaf,
         }
         return k;
     };
-    std::vector<uint8_t> root(ni, 0);
-    root[0] = 1;
-    std::vector<uint32_t> st{0};
+    // wide level of every internal node a cut refers to (root: 1).  The walk
+    // pushes at most three nodes per wide level above the deepest, so the
+    // greedy form is kept only while 3 * (levels - 1) fits the 96-entry
+    // traversal stack; otherwise the pairwise form is built (ceil(depth/2)
+    // levels: fits every tree of depth <= 64).
+    std::vector<uint32_t> level(ni, 0);
     Ref r[4];
-    while (!st.empty()) {
-        const uint32_t i = st.back();
-        st.pop_back();
-        const int k = cut(i, r);
-        for (int q = 0; q < k; ++q)
-            if (!r[q].leaf) { root[r[q].j] = 1; st.push_back(r[q].j); }
+    uint32_t wide_depth = 1;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        std::fill(level.begin(), level.end(), 0u);
+        level[0] = 1;
+        wide_depth = 1;
+        std::vector<uint32_t> st{0};
+        while (!st.empty()) {
+            const uint32_t i = st.back();
+            st.pop_back();
+            const int k = cut(i, r);
+            for (int q = 0; q < k; ++q)
+                if (!r[q].leaf) {
+                    level[r[q].j] = level[i] + 1;
+                    wide_depth = std::max(wide_depth, level[i] + 1);
+                    st.push_back(r[q].j);
+                }
+        }
+        if (3 * (wide_depth - 1) <= kStack) break;
+        if (!greedy) throw std::runtime_error("four-wide tree deeper than the traversal stack");
+        greedy = false;
     }
+    std::vector<uint8_t> root(ni, 0);
+    for (uint32_t i = 0; i < ni; ++i) root[i] = level[i] != 0;
+    s.stats.wide_depth = wide_depth;
+    s.stats.wide_form = greedy ? AURORA_BVH4_GREEDY : AURORA_BVH4_PAIRWISE;
     std::vector<uint32_t> widx(ni, 0xffffffffu);
     uint32_t nw = 0;
     for (uint32_t i = 0; i < ni; ++i)
@@ -438,6 +460,7 @@ void build_lbvh(Scene& s) {
         nd.parent = i == 0 ? 0xffffffffu : int_pos[int_parent[i]];
     }
     s.stats.depth = maxdepth;
+    if (maxdepth > kStack) throw std::runtime_error("LBVH deeper than the traversal stack");
     collapse_wide(s, gamma.data(), lleaf.data(), rleaf.data(), ni);
 }
 
@@ -546,7 +569,7 @@ Hit<R> closest_bvh(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt
         if (N[0].child & AURORA_NODE_LEAF) {
             test_prim(s, N[0].child & ~AURORA_NODE_LEAF, o, d, tmin, tmax, tn, best, cnt);
         } else {
-            uint32_t stack[64];
+            uint32_t stack[kStack];  // one push per level: depth <= kStack (checked at build)
             int sp = 0;
             uint32_t cur = N[0].child;
             for (;;) {
@@ -565,6 +588,7 @@ Hit<R> closest_bvh(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cnt
                     hb = false;
                 }
                 if (ha && hb) {
+                    if (sp >= int(kStack)) throw std::logic_error("binary traversal stack overflow");
                     if (tb < ta) { stack[sp++] = a.child; cur = b.child; }
                     else { stack[sp++] = b.child; cur = a.child; }
                 } else if (ha) {
@@ -641,7 +665,7 @@ Hit<R> closest_bvh4(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cn
     if (!s.built) throw std::logic_error("BVH not built");
     if (s.wide.empty()) { best.prim = AURORA_MISS; return best; }
     V3<R> inv = mk<R>(R(1) / d.x, R(1) / d.y, R(1) / d.z);
-    uint32_t stack[96];  // three pushes per level of a tree of depth <= 64
+    uint32_t stack[kStack];  // three pushes per wide level above the deepest (checked at build)
     int sp = 0;
     uint32_t cur = 0;
     for (;;) {
@@ -667,7 +691,7 @@ Hit<R> closest_bvh4(const Scene& s, V3<R> o, V3<R> d, R tmin, R tmax, Counts* cn
             continue;
         }
         for (int k = m - 1; k >= 1; --k) {
-            if (sp >= 96) throw std::logic_error("four-wide traversal stack overflow");
+            if (sp >= int(kStack)) throw std::logic_error("four-wide traversal stack overflow");
             stack[sp++] = w.slot[ord[k]].ref;
         }
         cur = w.slot[ord[0]].ref;

@@ -6,6 +6,7 @@
 //   are::reflect / are::refract       src/basic/vec3.cpp:182-199
 //   are::Plane::intersect_ray         src/basic/plane.cpp:13-27
 //   are::Texture fill ctor / pixel / paste (bilinear)  src/texture.cpp:52-83, 85-360
+//   are::Texture(path) P6 loader and save_texture       src/texture.cpp:9-50, 362-395
 // are::Triangle is not wrapped: its vtable needs Triangle::trace_texture,
 // which the reference declares (include/object/triangle.h:31) but never
 // defines, so it cannot be linked without a stand-in.
@@ -15,6 +16,8 @@
 #include <texture.h>
 
 #include <cmath>
+#include <cstring>
+#include <stdexcept>
 
 extern "C" {
 
@@ -64,6 +67,42 @@ void ref_paste(int sw, int sh, const double* src, int dw, int dh, const int* cor
             o[1] = c.y();
             o[2] = c.z();
         }
+}
+
+// load a P6 file with are::Texture(path): returns 0 and w, h and up to cap
+// texels (3 doubles each, rows top-down), or 1 and the exception's message
+int ref_load_ppm(const char* path, int* w, int* h, double* out, long cap, char* err, int errcap) {
+    try {
+        are::Texture t{std::string(path)};
+        *w = t.width_;
+        *h = t.height_;
+        long k = 0;
+        for (int y = 0; y < t.height_; ++y)
+            for (int x = 0; x < t.width_; ++x, ++k) {
+                if (k >= cap) return 0;
+                const are::Color3& c = t.pixel(x, y);
+                out[3 * k] = c.x();
+                out[3 * k + 1] = c.y();
+                out[3 * k + 2] = c.z();
+            }
+        return 0;
+    } catch (const std::exception& e) {
+        std::strncpy(err, e.what(), (size_t)errcap - 1);
+        err[errcap - 1] = 0;
+        return 1;
+    }
+}
+
+// write a w x h image (3 doubles per texel, rows top-down) with
+// are::Texture::save_texture; returns its result (1 written, 0 refused)
+int ref_save_ppm(int w, int h, const double* rgb, const char* path) {
+    are::Texture t(w, h, are::Color3(0, 0, 0));
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const double* c = rgb + 3 * ((size_t)y * w + x);
+            t.pixel(x, y) = are::Color3(c[0], c[1], c[2]);
+        }
+    return t.save_texture(path) ? 1 : 0;
 }
 
 }  // extern "C"

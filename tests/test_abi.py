@@ -28,7 +28,8 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert {e[0] for e in abi.EXPORTS} == set(names)
-    assert lib.aurora_abi_version() == 2
+    assert lib.aurora_abi_version() == abi.ABI_VERSION
+    assert f"#define AURORA_ABI_VERSION {abi.ABI_VERSION}" in open(HEADER).read()
 
 
 def test_struct_sizes_match_c():

@@ -38,12 +38,20 @@ def random_params(n, seed):
     return (rng.normal(size=n) * 0.05).astype(np.float32)
 
 
-def test_forward_matches_torch(ctx):
-    print("denoiser forward", flush=True)
+# Launch shapes (conv3x3_forward picks them from the pixel-block count of
+# each layer).  32x48: every layer has fewer than 128 blocks, so every conv
+# takes the channel-split form (one 16-channel tile per wave, NT = 1).
+# 1024x1024, the inference workload: levels 0-2 take the wide waves with
+# NT = 2, 3, 4 (32, 48, 64 output channels; 4096, 1024, 256 blocks), rows of
+# 16 and 4 segments of 64 pixels with their halos, encoder and concatenating
+# decoder forms; level 3 (128^2, 64 blocks) the channel split.
+@pytest.mark.parametrize("H,W", [(32, 48), (1024, 1024)])
+def test_forward_matches_torch(ctx, H, W):
+    print("denoiser forward", H, W, flush=True)
     net = dn.Denoiser(ctx, seed=1)
     p = random_params(net.n, 1)
     net.set_params(p)
-    H, W, spp = 32, 48, 4
+    spp = 4
     rgb, alb, nrm, _ = inputs(1, H, W, spp=spp)
     out = net.denoise(rgb[0].cuda(), alb[0].cuda(), nrm[0].cuda(), spp).cpu().numpy()
     out16, noisy_log = dn.torch_reference(torch.from_numpy(p), rgb, alb, nrm, spp)
@@ -54,14 +62,19 @@ def test_forward_matches_torch(ctx):
     assert err.mean() < 2e-3
 
 
+# 8 x 128^2 is the training step's batch: the forward and the data-gradient
+# convolutions take NT = 2 (level 0, 512 blocks) and NT = 3 (level 1, 128
+# blocks) and two row segments per 128-pixel row; levels 2-3 the channel
+# split; the weight gradient reduces 1024 partial blocks per tile.
+@pytest.mark.parametrize("B,H,W", [(2, 16, 32), (8, 128, 128)])
 @pytest.mark.parametrize("kind", ["log", "rel"])
-def test_gradients_match_torch_autograd(ctx, kind):
-    print("denoiser gradients", kind, flush=True)
+def test_gradients_match_torch_autograd(ctx, kind, B, H, W):
+    print("denoiser gradients", kind, B, H, W, flush=True)
     net = dn.Denoiser(ctx, seed=2)
     net.set_loss(kind)
     p = random_params(net.n, 2)
     net.set_params(p)
-    B, H, W, spp = 2, 16, 32, 4
+    spp = 4
     rgb, alb, nrm, tgt = inputs(B, H, W, seed=3, spp=spp)
     loss, g = net.loss_grad(rgb.cuda(), alb.cuda(), nrm.cuda(), tgt.cuda(), spp)
     pt = torch.from_numpy(p).to(torch.float64).requires_grad_(True)

@@ -31,6 +31,8 @@ SCENES = {
     "spheres": lambda: scenes.random_spheres(0),
     "stress20k": lambda: scenes.stress_scene(10000, 10000, seed=3),
     "stress200k": lambda: scenes.stress_scene(100000, 100000, seed=4),
+    # greedy collapse too deep for the traversal stack: the pairwise form
+    "deep_chain": lambda: scenes.deep_chain_scene(),
 }
 
 
@@ -39,8 +41,10 @@ def test_lbvh_bit_exact(ctx, name):
     say("lbvh", name)
     arr = SCENES[name]()
     g, gs, o, os_ = both(ctx, arr)
-    for k in ("n_prims", "n_nodes", "n_wide", "depth", "scene_lo", "scene_hi", "centroid_lo", "centroid_hi"):
+    for k in ("n_prims", "n_nodes", "n_wide", "depth", "wide_depth", "wide_form", "scene_lo", "scene_hi",
+              "centroid_lo", "centroid_hi"):
         assert gs[k] == os_[k], k
+    assert 3 * (gs["wide_depth"] - 1) <= 96
     gm, go = g.order()
     om, oo = o.order()
     assert np.array_equal(gm, om)
@@ -72,7 +76,8 @@ def dev(x):
     return torch.from_numpy(np.ascontiguousarray(x)).cuda()
 
 
-@pytest.mark.parametrize("name,nrays", [("cornell", 1 << 18), ("spheres", 1 << 18), ("stress200k", 1 << 20)])
+@pytest.mark.parametrize("name,nrays", [("cornell", 1 << 18), ("spheres", 1 << 18), ("stress200k", 1 << 20),
+                                       ("deep_chain", 1 << 16)])
 def test_closest_hit_bit_exact(ctx, name, nrays):
     say("trace", name, nrays)
     arr = SCENES[name]()
@@ -162,8 +167,10 @@ def test_full_size_stress_tree_and_hits(ctx):
     say("full-size stress tree")
     arr = scenes.stress_scene()
     g, gs, o, os_ = both(ctx, arr)
-    for k in ("n_prims", "n_nodes", "n_wide", "depth", "scene_lo", "scene_hi", "centroid_lo", "centroid_hi"):
+    for k in ("n_prims", "n_nodes", "n_wide", "depth", "wide_depth", "wide_form", "scene_lo", "scene_hi",
+              "centroid_lo", "centroid_hi"):
         assert gs[k] == os_[k], k
+    assert 3 * (gs["wide_depth"] - 1) <= 96
     assert gs["n_prims"] == 4_000_000
     assert np.array_equal(g.order()[1], o.order()[1])
     assert np.array_equal(g.nodes(), o.nodes())

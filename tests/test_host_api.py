@@ -14,9 +14,17 @@ PKG = os.path.join(ROOT, "aurora-rendering-engine_amd")
 EXE = os.path.join(PKG, "cornell_example")
 
 
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    """Builds libare.so and the example programs through host/Makefile (a
+    no-op when they are up to date)."""
+    subprocess.check_call(["make", "-s", "-C", os.path.join(PKG, "host"), "all"])
+
+
 def test_library_and_example_built():
     assert os.path.exists(os.path.join(PKG, "libare.so"))
     assert os.path.exists(EXE)
+    assert os.path.exists(os.path.join(PKG, "ppm_tool"))
 
 
 def test_flattened_cornell_matches_python_builder(tmp_path):
@@ -31,8 +39,11 @@ def test_flattened_cornell_matches_python_builder(tmp_path):
 @pytest.mark.gpu
 def test_cpp_render_matches_python_path(tmp_path, ctx):
     from aurora_amd.core import RenderSettings
+    from aurora_amd import abi
+    from oracle import orc
     out = tmp_path / "img.f32"
-    r = subprocess.run([EXE, "64", "8", str(out)], capture_output=True, text=True, timeout=120)
+    ppm = tmp_path / "img.ppm"
+    r = subprocess.run([EXE, "64", "8", str(out), str(ppm)], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     img = np.fromfile(out, np.float32).reshape(64, 64, 3)
@@ -40,6 +51,9 @@ def test_cpp_render_matches_python_path(tmp_path, ctx):
     g.build_bvh()
     rgb, _, _, _ = g.render(scenes.cornell_camera(64), RenderSettings(spp=8, max_depth=50))
     assert np.array_equal(img, rgb * np.float32(1.0 / 8))
+    # camera::render_ppm: a P6 file with the transfer of experiments/rt.cpp:377-392
+    body = orc.resolve_rgb8(rgb.reshape(-1), 8, abi.TRANSFER_GAMMA22).tobytes()
+    assert ppm.read_bytes() == b"P6\n64 64\n255\n" + body
     # the ray query of the example hits the same primitive as the Python trace
     prim, t = g.trace(np.array([[278, 278, -800]], np.float32), np.array([[0, 0, 1]], np.float32), 0.001)
     assert f"hit prim {prim[0]} " in r.stdout
