@@ -188,6 +188,8 @@ EXPORTS = [
      [_vp, C.POINTER(CameraDesc), C.POINTER(RenderDesc), _vp, _vp, _vp, C.POINTER(RenderStats)]),
     ("aurora_camera_rays", C.c_int, [_vp, C.POINTER(CameraDesc), _u32, _vp, _vp, _u32, _vp, _vp]),
     ("aurora_camera_rays_device", C.c_int, [_vp, C.POINTER(CameraDesc), _u32, _vp, _vp, _u32, _vp, _vp]),
+    ("aurora_paste", C.c_int, [_vp, _vp, _u32, _u32, _vp, _u32, _u32, _vp]),
+    ("aurora_paste_device", C.c_int, [_vp, _vp, _u32, _u32, _vp, _u32, _u32, _vp]),
     ("aurora_resolve_rgb8", C.c_int, [_vp, _u32, _u32, _u32, _u32, _vp]),
     ("aurora_resolve_rgb8_device", C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _vp]),
     ("aurora_render_batch_device", C.c_int, [_vp, C.POINTER(BatchItem), _u32, _u32, C.POINTER(RenderStats)]),

@@ -220,6 +220,24 @@ class Context:
                                                       samples.data_ptr(), pixels.shape[0], origins.data_ptr(),
                                                       dirs.data_ptr()))
 
+    def paste(self, src, dst, corners):
+        """are::Texture::paste on the GPU (host buffers): src (sh, sw, 3) is
+        warped into the quad corners = (LT, RT, LB, RB) (x, y pairs) of a
+        copy of dst (dh, dw, 3); returns the new destination."""
+        s = np.ascontiguousarray(src, np.float32)
+        d = np.array(dst, np.float32, copy=True)
+        c = np.ascontiguousarray(corners, np.int32).reshape(8)
+        self.check(self.lib.aurora_paste(self.h, s.ctypes.data, s.shape[1], s.shape[0], d.ctypes.data, d.shape[1],
+                                         d.shape[0], c.ctypes.data))
+        return d
+
+    def paste_device(self, src, dst, corners):
+        """Device form on float32 tensors (sh, sw, 3) and (dh, dw, 3); dst is
+        updated in place, asynchronously on the context stream."""
+        c = np.ascontiguousarray(corners, np.int32).reshape(8)
+        self.check(self.lib.aurora_paste_device(self.h, src.data_ptr(), src.shape[1], src.shape[0], dst.data_ptr(),
+                                                dst.shape[1], dst.shape[0], c.ctypes.data))
+
     def resolve_rgb8_device(self, rgb_sum, spp, out, transfer=abi.TRANSFER_GAMMA22):
         """8-bit resolve of a device radiance sum (H, W, 3) float32 into a
         uint8 tensor of the same shape.  Asynchronous on the context stream."""

@@ -297,6 +297,35 @@ def stress_camera(width=1920, height=1080):
     return Camera(width, height, 45.0, (0.5, 0.5, -1.2), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 0.0, 1.7)
 
 
+def cornell_triangles():
+    """The scene of host/examples/ingest_triangles.cpp as the ingestion of a
+    reference-style object set produces it: a 555-unit Cornell room, its
+    light and two axis-aligned boxes (one a mirror: metal of albedo 0.9, fuzz
+    0), each quad (Q, u, v) given as the triangles (Q, Q+u, Q+v) and
+    (Q+u+v, Q+v, Q+u) and ingested from those vertices: Q' = v0,
+    u' = v1 - v0, v' = v2 - v0 in double, then rounded to float."""
+    quads = [((555, 0, 0), (0, 555, 0), (0, 0, 555), "green"), ((0, 0, 0), (0, 555, 0), (0, 0, 555), "red"),
+             ((343, 554, 332), (-130, 0, 0), (0, 0, -105), "light"), ((0, 0, 0), (555, 0, 0), (0, 0, 555), "white"),
+             ((555, 555, 555), (-555, 0, 0), (0, 0, -555), "white"), ((0, 0, 555), (555, 0, 0), (0, 555, 0), "white")]
+
+    def box(a, b, m):
+        dx, dy, dz = b[0] - a[0], b[1] - a[1], b[2] - a[2]
+        return [((a[0], a[1], b[2]), (dx, 0, 0), (0, dy, 0), m), ((b[0], a[1], b[2]), (0, 0, -dz), (0, dy, 0), m),
+                ((b[0], a[1], a[2]), (-dx, 0, 0), (0, dy, 0), m), ((a[0], a[1], a[2]), (0, 0, dz), (0, dy, 0), m),
+                ((a[0], b[1], b[2]), (dx, 0, 0), (0, 0, -dz), m), ((a[0], a[1], a[2]), (dx, 0, 0), (0, 0, dz), m)]
+
+    quads += box((130, 0, 65), (295, 165, 230), "mirror") + box((265, 0, 295), (430, 330, 460), "diffuse")
+    b = Builder()
+    mats = {"red": b.lambertian((0.65, 0.05, 0.05)), "green": b.lambertian((0.12, 0.45, 0.15)),
+            "white": b.lambertian((0.73, 0.73, 0.73)), "light": b.light((15, 15, 15)),
+            "mirror": b.metal((0.9, 0.9, 0.9), 0.0), "diffuse": b.lambertian((0.73, 0.73, 0.73))}
+    for Q, u, v, m in quads:
+        Q, u, v = (np.array(x, np.float64) for x in (Q, u, v))
+        for v0, v1, v2 in ((Q, Q + u, Q + v), (Q + u + v, Q + v, Q + u)):
+            b.quad(tuple(v0), tuple(v1 - v0), tuple(v2 - v0), mats[m], kind=abi.QUAD_TRIANGLE)
+    return b.arrays()
+
+
 def deep_chain_scene(levels=30, cluster=4096):
     """A degenerate LBVH: a spine of `levels` binary levels, one per Morton
     bit (from the top), each peeling off a group of three coincident spheres

@@ -745,6 +745,18 @@ Status render_device(aurora_scene* s, const aurora_camera_desc* cam, const auror
     return render_jobs(s->ctx, jobs, stats, count_fetches, (rd->flags & AURORA_RENDER_GRAPH) != 0);
 }
 
+Status check_paste(const float* src, uint32_t sw, uint32_t sh, const float* dst, uint32_t dw, uint32_t dh,
+                   const int32_t* corners) {
+    if (!dst || !corners) return Status{AURORA_ERR_INVALID, "NULL destination or corners"};
+    if (dw == 0 || dh == 0) return Status{AURORA_ERR_INVALID, "destination width and height must be positive"};
+    if ((uint64_t)dw * dh > (1ull << 30) || (uint64_t)sw * sh > (1ull << 30))
+        return Status{AURORA_ERR_INVALID, "image too large"};
+    if (sw && sh && !src) return Status{AURORA_ERR_INVALID, "NULL source"};
+    if (sw > 0x7fffffffu || sh > 0x7fffffffu || dw > 0x7fffffffu || dh > 0x7fffffffu)
+        return Status{AURORA_ERR_INVALID, "image size beyond int range"};
+    return Status::ok();
+}
+
 }  // namespace
 
 // accessors for the other translation units of the C ABI (denoise.hip)
@@ -1144,6 +1156,38 @@ int aurora_camera_rays(aurora_ctx* c, const aurora_camera_desc* cam, uint32_t se
                                          b_d.as<float>(), c->stream);
     if (!r.bad()) r = hip_status(hipMemcpyAsync(org, b_o.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream), "copy");
     if (!r.bad()) r = hip_status(hipMemcpyAsync(dir, b_d.p, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream), "copy");
+    if (!r.bad()) r = hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    (void)hipStreamSynchronize(c->stream);
+    if (r.bad()) return fail(c, r);
+    return AURORA_OK;
+}
+
+
+int aurora_paste_device(aurora_ctx* c, const float* src, uint32_t sw, uint32_t sh, float* dst, uint32_t dw,
+                        uint32_t dh, const int32_t* corners) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    CK(c, check_paste(src, sw, sh, dst, dw, dh, corners));
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    bool painted = false;
+    CK(c, paste_device(src, sw, sh, dst, dw, dh, corners, c->stream, &painted));
+    return AURORA_OK;
+}
+
+int aurora_paste(aurora_ctx* c, const float* src, uint32_t sw, uint32_t sh, float* dst, uint32_t dw, uint32_t dh,
+                 const int32_t* corners) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    CK(c, check_paste(src, sw, sh, dst, dw, dh, corners));
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    const size_t sb = (size_t)sw * sh * 12, db = (size_t)dw * dh * 12;
+    DevBuf b_src, b_dst;
+    Status r = b_src.alloc(sb ? sb : 16);
+    if (!r.bad()) r = b_dst.alloc(db);
+    if (!r.bad() && sb) r = hip_status(hipMemcpyAsync(b_src.p, src, sb, hipMemcpyHostToDevice, c->stream), "copy");
+    if (!r.bad()) r = hip_status(hipMemcpyAsync(b_dst.p, dst, db, hipMemcpyHostToDevice, c->stream), "copy");
+    bool painted = false;
+    if (!r.bad()) r = paste_device(b_src.as<float>(), sw, sh, b_dst.as<float>(), dw, dh, corners, c->stream, &painted);
+    if (!r.bad() && painted)
+        r = hip_status(hipMemcpyAsync(dst, b_dst.p, db, hipMemcpyDeviceToHost, c->stream), "copy");
     if (!r.bad()) r = hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     (void)hipStreamSynchronize(c->stream);
     if (r.bad()) return fail(c, r);

@@ -140,6 +140,8 @@ Status launch_ray_keys(const float4* ray_o, const float4* ray_d, uint32_t n, con
                        uint32_t* vals, hipStream_t stream);
 int extend_block();         // threads per block of the extend kernel
 int extend_spill_levels();  // u32 stack entries per thread in ExtendArgs::spill
+constexpr int kExtendBlock = 256;        // extend_block()
+constexpr int kExtendSpillLevels = 80;   // extend_spill_levels()
 
 // Wavefront render.  A pass renders spp_pass samples of every pixel of one
 // or more scenes; each scene of the pass has a record in a device table and
@@ -184,6 +186,11 @@ Status launch_raygen(const Pass& p, Workspace& ws, hipStream_t s);
 // (pixel = j*W + i < W*H, checked by the caller)
 Status launch_camera_rays(const CamF& cam, uint32_t seed, const uint32_t* pixels, const uint32_t* samples,
                           uint32_t n, float* org, float* dir, hipStream_t s);
+// are::Texture::paste on the device: src (sw x sh RGB floats) warped into the
+// quad corners[0..7] = LT, RT, LB, RB (x, y) of dst (dw x dh); *painted is
+// false when there is nothing to paste (the reference's early returns)
+Status paste_device(const float* src, uint32_t sw, uint32_t sh, float* dst, uint32_t dw, uint32_t dh,
+                    const int32_t* corners, hipStream_t s, bool* painted);
 // 8-bit resolve of n floats (sum and out 16-byte and 4-byte aligned)
 Status launch_resolve_rgb8(const float* sum, size_t n, uint32_t spp, uint32_t transfer, uint8_t* out, hipStream_t s);
 Status launch_shade(const Pass& p, Workspace& ws, int cur, uint32_t depth, int grid, hipStream_t s);

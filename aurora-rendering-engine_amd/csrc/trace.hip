@@ -367,6 +367,7 @@ Status launch_ray_keys(const float4* ro, const float4* rd, uint32_t n, const flo
     return Status::ok();
 }
 
+static_assert(kExtendBlock == kBlock && kExtendSpillLevels == kMaxStack - kLds, "extend shape constants");
 int extend_block() { return kBlock; }
 int extend_spill_levels() { return kMaxStack - kLds; }
 

@@ -388,6 +388,26 @@ int aurora_resolve_rgb8_device(aurora_ctx* ctx, const float* d_rgb_sum,
                                uint32_t width, uint32_t height, uint32_t spp,
                                uint32_t transfer, uint8_t* d_out);
 
+/* are::Texture::paste (src/texture.cpp:85-360) on the GPU: warp src
+ * (src_w x src_h RGB floats, rows top-down) into dst (dst_w x dst_h) by the
+ * homography that sends the source corner pixels (0,0), (w-1,0), (0,h-1),
+ * (w-1,h-1) to corners[0..7] = left_top, right_top, left_bottom,
+ * right_bottom (x, y pixel positions).  A destination pixel whose centre
+ * lies inside the quad and maps inside the source takes the source's
+ * bilinear sample (clamped neighbours); every other pixel keeps its value.
+ * An empty source, a quad outside the destination or a degenerate mapping
+ * paste nothing and return AURORA_OK, as the reference returns.  The
+ * homography is solved on the host and the warp runs in double on the
+ * device, so each painted value is the reference's double result rounded
+ * to float.  Device buffers, asynchronous on the context stream. */
+int aurora_paste_device(aurora_ctx* ctx, const float* d_src, uint32_t src_w,
+                        uint32_t src_h, float* d_dst, uint32_t dst_w,
+                        uint32_t dst_h, const int32_t* corners);
+/* Same on host buffers (dst is read and written back). */
+int aurora_paste(aurora_ctx* ctx, const float* src, uint32_t src_w,
+                 uint32_t src_h, float* dst, uint32_t dst_w, uint32_t dst_h,
+                 const int32_t* corners);
+
 /* ---- learned denoiser ---------------------------------------------------
  * A 4-level U-Net (32/48/64/64 channels, 15 3x3 convolutions) on BF16 MFMA
  * with FP32 accumulation, fed the noisy radiance mean and the albedo and

@@ -66,6 +66,10 @@ int orc_camera(const aurora_camera_desc* c, float* out18);
  * aurora_camera_rays: 3*n origins and unit directions */
 int orc_camera_rays(const aurora_camera_desc* c, uint32_t seed, const uint32_t* pixels, const uint32_t* samples,
                     uint32_t n, float* org, float* dir);
+/* are::Texture::paste restated (src/texture.cpp:85-360): src (sw x sh RGB
+ * floats) warped into dst (dw x dh) by the homography of the source corners
+ * onto corners[0..7] = LT, RT, LB, RB (x, y); returns the pixels painted */
+int orc_paste(const float* src, int sw, int sh, float* dst, int dw, int dh, const int32_t* corners);
 /* 8-bit resolve of n floats (sum over spp samples), AURORA_TRANSFER_* */
 int orc_resolve_rgb8(const float* sum, uint64_t n, uint32_t spp, uint32_t transfer, uint8_t* out);
 /* pixels NULL = every pixel in row-major order; rgb/albedo/normal are 3 per

@@ -54,6 +54,7 @@ def lib():
             "orc_camera": (C.c_int, [C.POINTER(abi.CameraDesc), vp]),
             "orc_camera_rays": (C.c_int, [C.POINTER(abi.CameraDesc), u32, vp, vp, u32, vp, vp]),
             "orc_resolve_rgb8": (C.c_int, [vp, C.c_uint64, u32, u32, vp]),
+            "orc_paste": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, vp]),
             "orc_render_pixels": (C.c_int, [vp, C.POINTER(abi.CameraDesc), C.POINTER(abi.RenderDesc), vp, u32,
                                             C.c_int, vp, vp, vp, vp]),
             "orc_render_pixels_f64": (C.c_int, [vp, C.POINTER(abi.CameraDesc), C.POINTER(abi.RenderDesc), vp,
@@ -164,6 +165,16 @@ def resolve_rgb8(rgb_sum, spp, transfer):
     if rc:
         raise ValueError(_err())
     return out
+
+
+def paste(src, dst, corners):
+    """are::Texture::paste restated: returns (new dst, pixels painted).
+    src (sh, sw, 3) and dst (dh, dw, 3) float32; corners LT, RT, LB, RB."""
+    s = np.ascontiguousarray(src, np.float32)
+    d = np.array(dst, np.float32, copy=True)
+    c = np.ascontiguousarray(corners, np.int32).reshape(8)
+    n = lib().orc_paste(s.ctypes.data, s.shape[1], s.shape[0], d.ctypes.data, d.shape[1], d.shape[0], c.ctypes.data)
+    return d, n
 
 
 def hit_sphere(sphere, o, d, tmin=0.0, tmax=np.inf):
