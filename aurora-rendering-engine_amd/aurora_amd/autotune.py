@@ -5,7 +5,8 @@ shape, and the fastest is kept and cached under a signature of the
 workload (pixels, samples, primitives).  The shape covers the persistent
 traversal and shading grids (blocks per compute unit), the live-path count
 at which the fused tail takes over, whether passes are captured in a
-hipGraph, and which tree (binary or four-wide) the render kernels walk.
+hipGraph, and which tree (binary, four-wide, or the flat leaf array of a
+small scene) the render kernels walk.
 Shapes change speed only, never results (tests check the images bit for
 bit), so the tuner can be run at any time.
 """
@@ -21,8 +22,14 @@ CANDIDATES = {
     "shade_blocks_per_cu": [8],
     "tail_paths": [1 << 20, (1 << 32) - 1],
     "graph": [False, True],
-    "wide": [False, True],
+    "tree": [0, 1, 2],
 }
+
+
+def tree_of(shape):
+    """The render tree of a shape: its "tree" (0 binary, 1 four-wide, 2 flat),
+    or its "wide" flag."""
+    return int(shape["tree"]) if "tree" in shape else (1 if shape.get("wide", False) else 0)
 
 
 def apply(ctx, shape):
@@ -30,7 +37,7 @@ def apply(ctx, shape):
     defaults)."""
     ctx.set_launch(shape["extend_blocks_per_cu"], shape["shade_blocks_per_cu"], shape["tail_paths"])
     if hasattr(ctx, "set_render_tree"):
-        ctx.set_render_tree(bool(shape.get("wide", False)))
+        ctx.set_render_tree(tree_of(shape))
 
 
 def signature(items):

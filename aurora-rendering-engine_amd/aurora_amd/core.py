@@ -180,10 +180,12 @@ class Context:
     def set_launch(self, extend_blocks_per_cu=8, shade_blocks_per_cu=8, tail_paths=TAIL_ALWAYS):
         self.check(self.lib.aurora_set_launch(self.h, extend_blocks_per_cu, shade_blocks_per_cu, tail_paths))
 
-    def set_render_tree(self, wide=False):
-        """Render kernels walk the four-wide tree (True) or the binary LBVH
-        (False, default).  Speed only; the images are the same."""
-        self.check(self.lib.aurora_set_render_tree(self.h, 1 if wide else 0))
+    def set_render_tree(self, tree=0):
+        """Tree the render kernels walk: 0 (or False) the binary LBVH
+        (default), 1 (or True) the four-wide tree, 2 the flat leaf array of a
+        small scene (single-scene renders; four-wide elsewhere).  Speed only;
+        the images are the same."""
+        self.check(self.lib.aurora_set_render_tree(self.h, int(tree)))
 
     def set_ray_sort(self, mode=1):
         """Ray queries: 0 trace in the given order, 1 sort large queries on

@@ -38,7 +38,7 @@ struct aurora_ctx {
     uint32_t tail_paths = 0xffffffffu;  // fused-tail switch: live paths at or below (default: always)
     uint32_t graph_tail_depth = 8;   // captured passes switch to the fused tail here
     int ray_sort = 1;                // ray queries: 0 never sort, 1 large scenes, 2 always
-    uint32_t render_wide = 0;        // render bounces over the four-wide tree (else binary)
+    uint32_t render_tree = 0;        // render bounces: 0 binary LBVH, 1 four-wide tree, 2 flat (small scenes)
     Workspace ws;                    // path queues shared by the context's renders
     // pinned staging of the passes' scene tables, two in turn: a table is
     // rewritten only after the copy that read it two passes ago has run
@@ -54,7 +54,7 @@ struct aurora_scene {
     DevScene view{};
     uint32_t n_prims = 0;
     DevBuf spheres, sphere_mat, quads, quad_n, mats, texs, texels, perlin_vec, perlin_perm;
-    DevBuf nodes, wide, morton, order;
+    DevBuf nodes, wide, morton, order, flat;
     BvhScratch scr;
     bool built = false;
     aurora_bvh_stats stats{};
@@ -557,7 +557,10 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         }
         pa.recs = w.recs.as<SceneRec>();
         pa.n_recs = (uint32_t)jobs.size();
-        pa.wide = c->render_wide;
+        // the flat form needs one scene with a flat array; otherwise, and
+        // in the wavefront traversal kernels, the four-wide tree stands in
+        pa.wide = c->render_tree != 0 ? 1u : 0u;
+        pa.flat = c->render_tree == 2 && jobs.size() == 1 && jobs[0].scene->view.n_flat > 0 ? 1u : 0u;
         pa.n_paths = path_off;
         pa.n_pixels = pix_off;
         same_shape = sig == prev_sig;
@@ -585,7 +588,7 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         a.hit = w.hit.as<float2>();
         a.recs = w.recs.as<SceneRec>();
         a.one_scene = jobs.size() == 1;
-        a.wide = c->render_wide;  // the counting runs trace as the timed ones do
+        a.wide = c->render_tree != 0 ? 1u : 0u;  // the counting runs trace as the timed ones do
         a.totals = count_fetches ? jobs[0].scene->totals.as<unsigned long long>() : nullptr;
         a.spill = w.spill.as<uint32_t>();
         a.spill_stride = (uint32_t)(std::max(ext_grid, graph ? graph_tail_grid : tail_grid) * extend_block());
@@ -836,10 +839,11 @@ int aurora_set_launch(aurora_ctx* c, int ext_blocks_per_cu, int shade_blocks_per
     return AURORA_OK;
 }
 
-int aurora_set_render_tree(aurora_ctx* c, int wide) {
+int aurora_set_render_tree(aurora_ctx* c, int tree) {
     if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
-    if (wide != 0 && wide != 1) return fail(c, AURORA_ERR_INVALID, "render tree must be 0 (binary) or 1 (four-wide)");
-    c->render_wide = (uint32_t)wide;
+    if (tree < 0 || tree > 2)
+        return fail(c, AURORA_ERR_INVALID, "render tree must be 0 (binary), 1 (four-wide) or 2 (flat)");
+    c->render_tree = (uint32_t)tree;
     return AURORA_OK;
 }
 
@@ -927,7 +931,7 @@ void aurora_scene_destroy(aurora_scene* s) {
     if (!s) return;
     if (s->ctx && s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
     DevBuf* bufs[] = {&s->spheres, &s->sphere_mat, &s->quads, &s->quad_n, &s->mats, &s->texs, &s->texels, &s->perlin_vec,
-                      &s->perlin_perm, &s->nodes, &s->wide, &s->morton, &s->order, &s->rec1, &s->tr_o, &s->tr_d,
+                      &s->perlin_perm, &s->nodes, &s->wide, &s->morton, &s->order, &s->flat, &s->rec1, &s->tr_o, &s->tr_d,
                       &s->tr_hit, &s->tr_spill, &s->tr_pool, &s->totals, &s->tr_key[0], &s->tr_key[1],
                       &s->tr_val[0], &s->tr_val[1], &s->tr_hist};
     for (DevBuf* b : bufs) b->release();
@@ -945,6 +949,9 @@ int aurora_bvh_build(aurora_scene* s, aurora_bvh_stats* stats) {
     s->view.n_nodes = st.n_nodes;
     s->view.wide = s->wide.as<float4>();
     s->view.n_wide = st.n_nodes ? st.n_wide : 0u;
+    CK(c, flat_build(s->n_prims, s->nodes, s->scr, s->flat, c->stream));
+    s->view.flat = s->flat.as<float4>();
+    s->view.n_flat = s->n_prims <= kFlatMaxPrims ? s->n_prims : 0u;
     s->stats = st;
     s->built = true;
     if (stats) *stats = st;

@@ -211,7 +211,8 @@ struct DevScene {
     const int* perlin_perm;       // per slot: 3 x 256 ints
     const float4* nodes;          // 2 float4 per node: (lo, child) (hi, parent)
     const float4* wide;           // four-wide traversal nodes: 4 slots of (lo, ref) (hi, 0)
-    uint32_t n_spheres, n_quads, n_nodes, n_wide;
+    const float4* flat;           // small scenes: the leaf records in Morton order, (lo, leaf) (hi, parent)
+    uint32_t n_spheres, n_quads, n_nodes, n_wide, n_flat;
 };
 
 // Ray/sphere roots without the cancellations of the textbook quadratic

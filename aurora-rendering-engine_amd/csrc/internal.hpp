@@ -100,6 +100,12 @@ struct BvhScratch {
 Status lbvh_build(const DevScene& sc, uint32_t n_prims, DevBuf& nodes, DevBuf& wide, DevBuf& morton, DevBuf& order,
                   BvhScratch& scr, hipStream_t stream, aurora_bvh_stats* stats);
 
+// Scenes of at most kFlatMaxPrims primitives also keep their leaf records in
+// Morton order (32 bytes each) for the flat traversal; flat_build fills them
+// from the LBVH that lbvh_build left in `nodes` and `scr`.
+constexpr uint32_t kFlatMaxPrims = 256;
+Status flat_build(uint32_t n_prims, const DevBuf& nodes, const BvhScratch& scr, DevBuf& flat, hipStream_t stream);
+
 // Stable LSD radix sort of (key, value) pairs, 8 bits per pass from bit 0.
 // Sorted pairs end in (k0, v0) for an even number of passes, else in
 // (k1, v1).  hist holds radix_hist_bytes(n); after the call it holds the
@@ -169,6 +175,7 @@ struct Pass {
     uint32_t n_pixels;     // sum of WH over scenes with spp_pass > 0
     uint32_t any_aov;
     uint32_t wide;         // render traversal over the four-wide tree (else the binary one)
+    uint32_t flat;         // the fused tail walks the scene's flat leaf array (one scene with one)
 };
 struct Workspace {
     DevBuf q_o[2], q_d[2], q_b[2];   // path queues (float4 each)

@@ -395,6 +395,16 @@ __global__ __launch_bounds__(kBlock) void k_leaves(uint32_t n, const uint32_t* o
     nodes[2 * (size_t)pos + 1] = make_float4(bhi.x, bhi.y, bhi.z, __uint_as_float(int_pos[leaf_parent[m]]));
 }
 
+// the flat traversal array of a small scene: the leaf records in Morton order
+__global__ __launch_bounds__(kBlock) void k_flat_fill(uint32_t n, const uint32_t* leaf_pos, const float4* nodes,
+                                                      float4* flat) {
+    const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= n) return;
+    const size_t pos = n == 1 ? 0 : leaf_pos[m];
+    flat[2 * (size_t)m] = nodes[2 * pos];
+    flat[2 * (size_t)m + 1] = nodes[2 * pos + 1];
+}
+
 // the internal nodes of one depth: union of the two child records (written
 // by earlier launches), child index, parent position
 __global__ __launch_bounds__(kBlock) void k_refit_level(const uint32_t* lvl_node, uint32_t count,
@@ -667,6 +677,15 @@ Status radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, 
         scan_exclusive(hist, 256u * nb, st);
         k_scatter<<<nb, kBlock, 0, st>>>(ks[a], vs[a], ks[b], vs[b], n, 8 * pass, hist, nb);
     }
+    AUR_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status flat_build(uint32_t n, const DevBuf& nodes, const BvhScratch& s, DevBuf& flat, hipStream_t st) {
+    if (n == 0 || n > kFlatMaxPrims) return Status::ok();
+    AUR_TRY(flat.alloc((size_t)n * 32));
+    k_flat_fill<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.leaf_pos.as<uint32_t>(), nodes.as<float4>(),
+                                                          flat.as<float4>());
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }

@@ -343,10 +343,11 @@ constexpr int kTailRefill = 4;
 constexpr uint32_t kTailBatchMin = 64, kTailBatchMax = 128;
 
 // five waves per SIMD: the register budget this leaves (84) holds the kernel
-// without spills; the resident grid (five blocks per CU) matches it.  WIDE:
-// traverse the four-wide tree, else the binary one (the faster form depends
-// on the scene, so the launch shape chooses).
-template <bool ONE, bool WIDE>
+// without spills; the resident grid (five blocks per CU) matches it.  TREE:
+// 1 traverse the four-wide tree, 0 the binary one, 2 (single-scene passes of
+// small scenes) the flat leaf array (the faster form depends on the scene,
+// so the launch shape chooses).
+template <bool ONE, int TREE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_tail(
     Pass pa, ShadeBufs b, uint32_t depth0, uint32_t* spill,
                                                  uint32_t spill_stride) {
@@ -423,7 +424,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         ++segs;
         float t;
         int prim;
-        if constexpr (WIDE)
+        if constexpr (TREE == 2)
+            traverse_closest_flat(r.sc, org, dir, r.tmin, INFINITY, t, prim);
+        else if constexpr (TREE == 1)
             traverse_closest_wide(r.sc, org, dir, r.tmin, INFINITY, stk + threadIdx.x, kBlock, my_spill,
                                   spill_stride, t, prim);
         else
@@ -556,8 +559,8 @@ Status launch_tail(const Pass& p, Workspace& ws, int cur, uint32_t depth, int gr
     b.no = b.nd = b.nb = nullptr;
     const size_t need = (size_t)grid * kBlock * (kTravMaxStack - kTravLds) * 4;
     if (need > ws.spill.bytes) return Status{AURORA_ERR_RUNTIME, "tail grid exceeds the stack spill buffer"};
-    auto k = p.n_recs == 1 ? (p.wide ? k_tail<true, true> : k_tail<true, false>)
-                           : (p.wide ? k_tail<false, true> : k_tail<false, false>);
+    auto k = p.n_recs == 1 ? (p.flat ? k_tail<true, 2> : p.wide ? k_tail<true, 1> : k_tail<true, 0>)
+                           : (p.wide ? k_tail<false, 1> : k_tail<false, 0>);
     k<<<grid, kBlock, 0, s>>>(p, b, depth, ws.spill.as<uint32_t>(), (uint32_t)(grid * kBlock));
     AUR_HIP(hipGetLastError());
     return Status::ok();

@@ -167,6 +167,49 @@ __device__ __forceinline__ void traverse_closest_wide(const DevScene& sc, v3 o, 
     prim_out = miss ? AURORA_MISS : (int)best_id;
 }
 
+// Flat form, for scenes of a few primitives: every leaf record in Morton
+// order, its box's slab test against the current best t and its primitive
+// test, the same sequence for every ray.  Within a single-scene launch the
+// records sit at wave-uniform addresses (scalar loads) and no lane waits for
+// another's traversal: no stack, no divergence.  A primitive is taken under
+// the same conditions as in the tree walks (its leaf box passes the slab
+// test at the current best t, the hit lies at or beyond the box's entry
+// distance, nearer or an equal t with a lower id), so the closest hit is the
+// tree walks' one.
+__device__ __forceinline__ void traverse_closest_flat(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
+                                                      float& t_out, int& prim_out) {
+    const float4* const F = sc.flat;
+    const float4* const S = sc.spheres;
+    const float4* const Qd = sc.quads;
+    const uint32_t n_spheres = sc.n_spheres, n_flat = sc.n_flat;
+    const uint32_t n_prims = sc.n_spheres + sc.n_quads;
+    const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    float best_t = tmax;
+    uint32_t best_id = 0x7fffffffu;
+    for (uint32_t k = 0; k < n_flat; ++k) {
+        const float4 lo = ldg4(F, 2 * (size_t)k), hi = ldg4(F, 2 * (size_t)k + 1);
+        float tn = 0.0f;
+        const bool hb = slab(lo, hi, o, inv, tmin, best_t, tn);
+        const uint32_t id = __float_as_uint(lo.w) & ~AURORA_NODE_LEAF;
+        const bool valid = id < n_prims;
+        const uint32_t q = valid ? id : 0u;
+        float t = 0.0f;
+        bool h;
+        if (q < n_spheres) {
+            h = hit_sphere_t(ldg4(S, q), o, d, tmin, tmax, t);
+        } else {
+            const size_t b = 3 * (size_t)(q - n_spheres);
+            h = hit_quad_t(ldg4(Qd, b), ldg4(Qd, b + 1), ldg4(Qd, b + 2), o, d, tmin, tmax, t);
+        }
+        const bool take = hb & valid & h & (t >= tn) & ((t < best_t) | ((t == best_t) & (id < best_id)));
+        best_t = take ? t : best_t;
+        best_id = take ? id : best_id;
+    }
+    const bool miss = best_id == 0x7fffffffu;
+    t_out = miss ? tmax : best_t;
+    prim_out = miss ? AURORA_MISS : (int)best_id;
+}
+
 // Binary form: one step fetches the aligned child pair at `cur` (two node
 // records, 64 bytes), tests both slabs, tests leaf primitives at once and
 // descends to the nearer child, pushing the farther.  Used where traversal

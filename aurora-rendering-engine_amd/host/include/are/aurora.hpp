@@ -288,6 +288,38 @@ class bvh_node : public hittable {
     std::vector<std::shared_ptr<hittable>> objects;
 };
 
+// ------------------------------------------------------- file formats -----
+// Portable FloatMap (PFM): FP32 radiance or AOV images without loss.  Header
+// "PF\n<w> <h>\n-1.0\n" (colour, negative scale = little-endian floats),
+// then the scanlines from the bottom row up; in memory rows run top-down
+// (width*height*3 floats, the layout of aurora_render's sums).  read_pfm
+// also takes big-endian files (positive scale).  Errors throw
+// std::runtime_error naming the file, as are::Texture's PPM loader does
+// (src/texture.cpp:11-43).
+void write_pfm(const std::string& path, int width, int height, const std::vector<float>& rgb);
+std::vector<float> read_pfm(const std::string& path, int& width, int& height);
+
+// Checkpoint of a progressive render: the camera and render settings, the
+// number of samples accumulated (samples [render.sample_offset,
+// render.sample_offset + spp) of every pixel) and the per-pixel sums
+// (width*height*3 floats each, rows top-down; albedo/normal empty without
+// AOVs).  Resuming adds the following samples in sample order, so stop ->
+// save -> load -> resume gives the sums of an uninterrupted render bit for
+// bit.  File: magic "AURACC01", a 128-byte little-endian header (u32 header
+// size, width, height, flags (1: AOV sums), camera vfov, lookfrom, lookat,
+// vup, defocus_angle, focus_dist (f32), u32 spp, sample_offset, max_depth,
+// seed, f32 background[3], tmin, zero padding, u32 CRC-32 (zlib polynomial)
+// of the payload, u32 0), then the f32 sums: rgb, then albedo and normal.  load() checks
+// the magic, sizes, trailing bytes and checksum (std::runtime_error).
+struct accumulation {
+    aurora_camera_desc camera{};
+    aurora_render_desc render{};  // max_depth, seed, background, tmin, sample_offset
+    uint32_t spp = 0;             // samples accumulated so far
+    std::vector<float> rgb_sum, albedo_sum, normal_sum;
+    void save(const std::string& path) const;
+    static accumulation load(const std::string& path);
+};
+
 // ----------------------------------------------------------- camera -----
 class camera {
    public:
@@ -314,6 +346,12 @@ class camera {
     void render(const hittable& world, std::vector<float>& out);
     // Render and write a binary P6 PPM (layout of src/texture.cpp:370-391).
     void render_ppm(const hittable& world, const std::string& path);
+    // Progressive rendering: add `more` samples to acc.  An empty acc
+    // (spp 0, no sums) starts from this camera's settings (aovs: also sum the
+    // first-hit albedo and normal); a loaded one must have been made with
+    // the same settings (std::invalid_argument otherwise).  Samples
+    // [sample_offset + spp, sample_offset + spp + more) are added.
+    void accumulate(const hittable& world, accumulation& acc, int more, bool aovs = false);
     int image_height() const;
     aurora_render_stats last_stats{};
 

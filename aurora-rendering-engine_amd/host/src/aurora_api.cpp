@@ -5,6 +5,7 @@
 #include "are/aurora.hpp"
 
 #include <cstdio>
+#include <cstring>
 #include <iostream>
 #include <mutex>
 #include <stdexcept>
@@ -295,6 +296,57 @@ bool hittable::hit(const ray& r, interval ray_t, hit_record& rec) const {
 int camera::image_height() const {
     int h = (int)(image_width / aspect_ratio);
     return h < 1 ? 1 : h;
+}
+
+void camera::accumulate(const hittable& world, accumulation& acc, int more, bool aovs) {
+    if (more < 0) throw std::invalid_argument("sample count must be non-negative");
+    const int h = image_height();
+    aurora_camera_desc cd{};
+    cd.image_width = (uint32_t)image_width;
+    cd.image_height = (uint32_t)h;
+    cd.vfov = (float)vfov;
+    put3(cd.lookfrom, lookfrom);
+    put3(cd.lookat, lookat);
+    put3(cd.vup, vup);
+    cd.defocus_angle = (float)defocus_angle;
+    cd.focus_dist = (float)focus_dist;
+    aurora_render_desc rd{};
+    rd.max_depth = (uint32_t)max_depth;
+    rd.seed = seed;
+    put3(rd.background, background);
+    const size_t n = (size_t)image_width * h * 3;
+    if (acc.spp == 0 && acc.rgb_sum.empty()) {
+        acc.camera = cd;
+        acc.render = rd;
+        acc.rgb_sum.assign(n, 0.0f);
+        if (aovs) {
+            acc.albedo_sum.assign(n, 0.0f);
+            acc.normal_sum.assign(n, 0.0f);
+        }
+    } else {
+        if (std::memcmp(&acc.camera, &cd, sizeof cd) != 0)
+            throw std::invalid_argument("accumulation: made with another camera");
+        const aurora_render_desc& r = acc.render;
+        if (r.max_depth != rd.max_depth || r.seed != rd.seed ||
+            std::memcmp(r.background, rd.background, sizeof rd.background) != 0)
+            throw std::invalid_argument("accumulation: made with other render settings");
+        if (acc.rgb_sum.size() != n) throw std::invalid_argument("accumulation: sums do not match the image");
+    }
+    const bool aov = !acc.albedo_sum.empty();
+    if (aov && (acc.albedo_sum.size() != n || acc.normal_sum.size() != n))
+        throw std::invalid_argument("accumulation: AOV sums do not match the image");
+    if ((uint64_t)acc.render.sample_offset + acc.spp + (uint64_t)more > 0xffffffffull)
+        throw std::invalid_argument("accumulation: sample index overflow");
+    if (more == 0) return;
+    auto d = world.device();
+    aurora_render_desc go = acc.render;
+    go.spp = (uint32_t)more;
+    go.sample_offset = acc.render.sample_offset + acc.spp;
+    go.flags = aov ? AURORA_RENDER_AOV : 0u;
+    go.spp_per_pass = 0;
+    check(d->ctx, aurora_render(d->scene, &acc.camera, &go, acc.rgb_sum.data(), aov ? acc.albedo_sum.data() : nullptr,
+                                aov ? acc.normal_sum.data() : nullptr, &last_stats));
+    acc.spp += (uint32_t)more;
 }
 
 void camera::render_sum(const hittable& world, std::vector<float>& sum) {

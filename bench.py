@@ -314,7 +314,7 @@ def main():
     ap.add_argument("--no-metric", action="store_true", help="skip time-to-relMSE (plain vs denoised)")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving section")
     ap.add_argument("--no-denoiser", action="store_true", help="skip the denoiser throughput section")
-    ap.add_argument("--render-tree", choices=["binary", "wide"], default="binary",
+    ap.add_argument("--render-tree", choices=["binary", "wide", "flat"], default="binary",
                     help="render traversal tree when --tail-paths fixes the shape (no autotuning)")
     ap.add_argument("--only", choices=["stress", "copy", "serving", "denoiser"], default=None,
                     help="profiling modes: the stress section alone, or a plain 1 GiB device copy "
@@ -381,12 +381,13 @@ def main():
         best, tune_table = autotune.tune(ctx, tune_run, f"cornell_{W}x{H}_spp{S}", cache_path=None, log=note,
                                          candidates={"extend_blocks_per_cu": [8], "shade_blocks_per_cu": [8],
                                                      "tail_paths": [1 << 20, TAIL_ALWAYS],
-                                                     "graph": [False, True], "wide": [False, True]})
+                                                     "graph": [False, True], "tree": [0, 1, 2]})
         graph = best["graph"]
         shape = {k: best[k] for k in best if k != "seconds"}
     else:
-        ctx.set_render_tree(args.render_tree == "wide")
-        shape = {"tail_paths": args.tail_paths, "wide": args.render_tree == "wide"}
+        tree = {"binary": 0, "wide": 1, "flat": 2}[args.render_tree]
+        ctx.set_render_tree(tree)
+        shape = {"tail_paths": args.tail_paths, "tree": tree}
 
     if args.sweep_tail:
         for tp in (0, 262144, 1 << 20, 4 << 20, TAIL_ALWAYS):

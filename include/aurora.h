@@ -236,9 +236,13 @@ int aurora_set_launch(aurora_ctx* ctx, int extend_blocks_per_cu,
 
 /* Tree walked by the render kernels: 0 = the binary LBVH (default; one
  * 64-byte child pair per step), 1 = the four-wide tree (one 128-byte node
- * per step).  Which is faster depends on the scene (the autotuner tries
- * both); the images are the same.  No reference counterpart. */
-int aurora_set_render_tree(aurora_ctx* ctx, int wide);
+ * per step), 2 = flat: for single-scene renders of scenes of at most 256
+ * primitives, the persistent path kernel tests every leaf in Morton order,
+ * the same sequence for every ray (no stack, no divergence); elsewhere 2
+ * walks the four-wide tree.  Which is fastest depends on the scene (the
+ * autotuner tries them); the images are the same.  No reference
+ * counterpart. */
+int aurora_set_render_tree(aurora_ctx* ctx, int tree);
 
 /* Order of the rays of aurora_trace_closest*: 0 = as given, 1 = sorted by
  * origin (Morton code in the root box) and direction octant when the query

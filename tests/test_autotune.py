@@ -9,13 +9,13 @@ from aurora_amd import autotune
 class FakeCtx:
     def __init__(self):
         self.shape = None
-        self.wide = None
+        self.tree = None
 
     def set_launch(self, e, s, t):
         self.shape = (e, s, t)
 
-    def set_render_tree(self, wide):
-        self.wide = wide
+    def set_render_tree(self, tree):
+        self.tree = tree
 
 
 def test_picks_cheapest_and_caches(tmp_path, monkeypatch):
@@ -23,20 +23,21 @@ def test_picks_cheapest_and_caches(tmp_path, monkeypatch):
     calls = []
     clock = [0.0]
 
-    def cost(shape, graph, wide):
+    def cost(shape, graph, tree):
         e, s, t = shape
-        return 1.0 + abs(e - 8) + (0.5 if t != 1 << 20 else 0.0) + (0.0 if graph else 0.25) + (0.1 if wide else 0.0)
+        return (1.0 + abs(e - 8) + (0.5 if t != 1 << 20 else 0.0) + (0.0 if graph else 0.25) +
+                {0: 0.1, 1: 0.2, 2: 0.0}[tree])
 
     def run(graph):
         calls.append((ctx.shape, graph))
-        clock[0] += cost(ctx.shape, graph, ctx.wide)
+        clock[0] += cost(ctx.shape, graph, ctx.tree)
 
     monkeypatch.setattr(autotune.time, "perf_counter", lambda: clock[0])
     cache = tmp_path / "tune.json"
     best, table = autotune.tune(ctx, run, "sigA", cache_path=str(cache))
-    assert (best["extend_blocks_per_cu"], best["tail_paths"], best["graph"]) == (8, 1 << 20, True)
-    assert ctx.shape == (8, 8, 1 << 20) and ctx.wide is False
-    n_cand = 3 * 1 * 2 * 2 * 2
+    assert (best["extend_blocks_per_cu"], best["tail_paths"], best["graph"], best["tree"]) == (8, 1 << 20, True, 2)
+    assert ctx.shape == (8, 8, 1 << 20) and ctx.tree == 2
+    n_cand = 3 * 1 * 2 * 2 * 3
     assert len(table) == n_cand and len(calls) == n_cand * 3  # warm-up + 2 repeats each
     saved = json.loads(cache.read_text())
     assert saved["sigA"]["best"]["graph"] is True
@@ -44,7 +45,12 @@ def test_picks_cheapest_and_caches(tmp_path, monkeypatch):
     ctx2 = FakeCtx()
     calls.clear()
     best2, _ = autotune.tune(ctx2, run, "sigA", cache_path=str(cache))
-    assert not calls and ctx2.shape == (8, 8, 1 << 20) and best2 == best
+    assert not calls and ctx2.shape == (8, 8, 1 << 20) and ctx2.tree == 2 and best2 == best
+
+
+def test_tree_of_reads_old_and_new_shapes():
+    assert autotune.tree_of({"tree": 2}) == 2
+    assert autotune.tree_of({"wide": True}) == 1 and autotune.tree_of({"wide": False}) == 0 and autotune.tree_of({}) == 0
 
 
 def test_signature_separates_workloads():

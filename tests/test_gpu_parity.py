@@ -286,8 +286,9 @@ def test_render_radiance_bit_exact(ctx, name, mk, mkcam, rs):
     g, gs, o, os_ = both(ctx, arr)
     orgb, _, _, segs = o.render(cam, rs)
     # pure wavefront, wavefront then fused tail, fused from the first bounce;
-    # binary and four-wide render trees
-    for tail, wide in ((0, False), (2048, False), (1 << 30, False), (0, True), (1 << 30, True)):
+    # binary, four-wide and flat render trees (flat: scenes of at most 256
+    # primitives in the fused tail; the four-wide tree elsewhere)
+    for tail, wide in ((0, 0), (2048, 0), (1 << 30, 0), (0, 1), (1 << 30, 1), (2048, 2), (1 << 30, 2)):
         ctx.set_launch(tail_paths=tail)
         ctx.set_render_tree(wide)
         try:
@@ -314,13 +315,16 @@ def test_render_textures_bit_exact(ctx):
     rs = RenderSettings(spp=4, max_depth=20, background=(0.6, 0.7, 0.9), aov=True)
     orgb, oalb, onrm, segs = o.render(cam, rs)
     assert len(np.unique(oalb.reshape(-1, 3), axis=0)) > 500  # the textures vary across the image
-    for tail in (0, (1 << 32) - 1):
+    assert arr.n_prims <= 256  # small enough for the flat form
+    for tail, tree in ((0, 0), ((1 << 32) - 1, 0), ((1 << 32) - 1, 1), ((1 << 32) - 1, 2)):
         ctx.set_launch(tail_paths=tail)
+        ctx.set_render_tree(tree)
         try:
             grgb, galb, gnrm, st = g.render(cam, rs)
         finally:
             ctx.set_launch()
-        say("textures tail", tail, "segments gpu/cpu", st["segments"], segs)
+            ctx.set_render_tree(0)
+        say("textures tail", tail, "tree", tree, "segments gpu/cpu", st["segments"], segs)
         assert st["segments"] == segs
         assert np.array_equal(grgb.view(np.uint32), orgb.view(np.uint32))
         assert np.array_equal(galb.view(np.uint32), oalb.view(np.uint32))
@@ -334,12 +338,14 @@ def test_render_aov_and_progressive(ctx):
     g, gs, o, os_ = both(ctx, arr)
     rs = RenderSettings(spp=6, aov=True, spp_per_pass=4)
     orgb, oalb, onrm, _ = o.render(cam, rs)
-    for tail in (0, 1 << 30):
+    for tail, tree in ((0, 0), (1 << 30, 0), (1 << 30, 2)):
         ctx.set_launch(tail_paths=tail)
+        ctx.set_render_tree(tree)
         try:
             grgb, galb, gnrm, _ = g.render(cam, rs)
         finally:
             ctx.set_launch()
+            ctx.set_render_tree(0)
         assert np.array_equal(grgb, orgb)
         assert np.array_equal(galb, oalb)
         assert np.array_equal(gnrm, onrm)
