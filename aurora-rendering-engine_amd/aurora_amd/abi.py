@@ -26,6 +26,8 @@ BVH4_EMPTY = 0xFFFFFFFF
 BVH4_PAIRWISE = 0
 BVH4_GREEDY = 1
 TRAVERSAL_STACK = 96  # entries of the device traversal stack
+NOALLOC_MAX_RAYS = 262144  # AURORA_NOALLOC_MAX_RAYS
+NOALLOC_SCRATCH_BYTES = 52430848  # AURORA_NOALLOC_SCRATCH_BYTES
 TEX_SOLID = 0
 TEX_CHECKER = 1
 TEX_NOISE = 2
@@ -179,6 +181,7 @@ EXPORTS = [
     ("aurora_bvh_download_order", C.c_int, [_vp, _vp, _vp, _u32]),
     ("aurora_trace_closest", C.c_int, [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp]),
     ("aurora_trace_closest_device", C.c_int, [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp, _vp]),
+    ("aurora_trace_closest_device_noalloc", C.c_int, [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp, _vp]),
     ("aurora_trace_brute_device", C.c_int, [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp]),
     ("aurora_trace_closest_device_stats", C.c_int,
      [_vp, _vp, _vp, _u32, _f, _f, _vp, _vp, C.c_int, C.POINTER(TraceStats)]),

@@ -320,6 +320,14 @@ class Scene:
                                                       counts.data_ptr() if counts is not None else None)
         self.ctx.check(rc)
 
+    def trace_device_noalloc(self, o, d, prim, t, tmin=0.0, tmax=float("inf"), counts=None):
+        """aurora_trace_closest_device_noalloc: as trace_device, with every
+        intermediate buffer in the library's static per-device scratch array
+        (at most abi.NOALLOC_MAX_RAYS rays).  Asynchronous."""
+        self.ctx.check(self.lib.aurora_trace_closest_device_noalloc(
+            self.h, o.data_ptr(), d.data_ptr(), o.shape[0], tmin, tmax, prim.data_ptr(), t.data_ptr(),
+            counts.data_ptr() if counts is not None else None))
+
     def trace_device_timed(self, o, d, prim, t, tmin=0.0, tmax=float("inf"), count_fetches=False):
         """Device trace with the traversal kernel timed by HIP events;
         returns the stats dict (extend_ms, fetch totals when counted)."""

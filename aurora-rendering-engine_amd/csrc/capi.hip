@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,7 @@ struct aurora_ctx {
     hipEvent_t recs_ev[2] = {nullptr, nullptr};
     int recs_turn = 0;
     uint32_t host_recs_cap = 0;
+    unsigned char* scratch = nullptr;  // this device's copy of the allocation-free scratch array
 };
 
 struct aurora_scene {
@@ -63,7 +65,64 @@ struct aurora_scene {
     DevBuf tr_key[2], tr_val[2], tr_hist;  // ray sort of large queries
 };
 
+// ---- allocation-free ray queries ----------------------------------------
+// aurora_trace_closest_device_noalloc keeps every buffer beyond the caller's
+// rays and hits in `scratch`, a static device array of which each GPU holds
+// its own copy until the process ends: the packed rays and hits of up to
+// kNoAllocMaxRays rays, the work-pool counters, the one-entry scene table and
+// the traversal stack spill of a grid of at most kNoAllocGrid blocks.  Host
+// code never passes the array's name to a kernel (on the host it is only a
+// stand-in): its device address is taken once per device with
+// hipGetSymbolAddress (scratch_address, called by aurora_create) and kept
+// in the context.
+constexpr uint32_t kNoAllocMaxRays = AURORA_NOALLOC_MAX_RAYS;
+constexpr int kNoAllocGrid = 512;  // two 256-thread blocks per CU
+constexpr size_t kNaRayBytes = (size_t)kNoAllocMaxRays * 16;  // float4 per ray, origins then directions
+constexpr size_t kNaHitBytes = (size_t)kNoAllocMaxRays * 8;   // float2 (t, prim) per ray
+constexpr size_t kNaPoolBytes = (size_t)kPools * kPoolStride * 4;
+constexpr size_t kNaRecBytes = 1024;
+constexpr size_t kNaSpillBytes = (size_t)kNoAllocGrid * kExtendBlock * kExtendSpillLevels * 4;
+constexpr size_t kNaOffDir = kNaRayBytes;
+constexpr size_t kNaOffHit = 2 * kNaRayBytes;
+constexpr size_t kNaOffPool = kNaOffHit + kNaHitBytes;
+constexpr size_t kNaOffRec = kNaOffPool + kNaPoolBytes;
+constexpr size_t kNaOffSpill = kNaOffRec + kNaRecBytes;
+constexpr size_t kNoAllocScratchBytes = kNaOffSpill + kNaSpillBytes;
+static_assert(kNoAllocScratchBytes == AURORA_NOALLOC_SCRATCH_BYTES, "scratch size stated in aurora.h");
+static_assert(sizeof(SceneRec) <= kNaRecBytes, "scene record fits its slot");
+static_assert(kNaOffDir % 256 == 0 && kNaOffHit % 256 == 0 && kNaOffPool % 256 == 0 && kNaOffRec % 256 == 0 &&
+                  kNaOffSpill % 256 == 0,
+              "scratch regions 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+unsigned char* g_scratch_dev[kMaxDevices] = {};
+
+// the device address of this device's scratch array; `device` must be the
+// current device
+Status scratch_address(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return Status{AURORA_ERR_INVALID, "device ordinal beyond 63"};
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        AUR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_dev[device];
+    return Status::ok();
+}
+
+// clears the work-pool counters and writes the one-entry scene table of an
+// allocation-free query (a kernel, not a copy or a memset, so the call stays
+// capturable)
+__global__ void k_noalloc_setup(uint32_t* pool, SceneRec* rec, SceneRec value) {
+    const uint32_t t = threadIdx.x;
+    if (t < (uint32_t)(kPools * kPoolStride)) pool[t] = 0u;
+    if (t == 0) *rec = value;
+}
 
 thread_local std::string g_create_err;
 
@@ -796,6 +855,14 @@ int aurora_create(int device, aurora_ctx** out) {
         delete c;
         return fail(nullptr, s);
     }
+    if (!s.bad()) s = scratch_address(device, &c->scratch);
+    if (s.bad()) {
+        for (int k = 0; k < 2; ++k)
+            if (c->recs_ev[k]) (void)hipEventDestroy(c->recs_ev[k]);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(nullptr, s);
+    }
     c->cus = prop.multiProcessorCount;
     c->ext_grid = c->cus * 8;
     c->shade_grid = c->cus * 8;
@@ -1020,6 +1087,54 @@ int aurora_trace_closest_device_stats(aurora_scene* s, const float* o, const flo
     CK(s->ctx, trace_device(s, o, d, n, tmin, tmax, prim, t, nullptr, false, &local, count_fetches != 0));
     CK(s->ctx, hip_status(hipStreamSynchronize(s->ctx->stream), "hipStreamSynchronize"));
     if (stats) *stats = local;
+    return AURORA_OK;
+}
+
+int aurora_trace_closest_device_noalloc(aurora_scene* s, const float* o, const float* d, uint32_t n, float tmin,
+                                        float tmax, int32_t* prim, float* t, uint32_t* counts) {
+    if (!s) return fail(nullptr, AURORA_ERR_INVALID, "scene is NULL");
+    aurora_ctx* c = s->ctx;
+    if (n > kNoAllocMaxRays)
+        return fail(c, AURORA_ERR_INVALID,
+                    "aurora_trace_closest_device_noalloc: " + std::to_string(n) +
+                        " rays exceed the allocation-free maximum of " + std::to_string(kNoAllocMaxRays) +
+                        " per call");
+    if (!s->built) return fail(c, AURORA_ERR_STATE, "BVH not built: call aurora_bvh_build first");
+    if (n == 0) return AURORA_OK;
+    if (!o || !d || !prim || !t) return fail(c, AURORA_ERR_INVALID, "NULL buffer");
+    if (!c->scratch) return fail(c, AURORA_ERR_STATE, "no scratch address for this context's device");
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    unsigned char* base = c->scratch;
+    float4* ro = reinterpret_cast<float4*>(base);
+    float4* rd = reinterpret_cast<float4*>(base + kNaOffDir);
+    float2* hit = reinterpret_cast<float2*>(base + kNaOffHit);
+    uint32_t* pool = reinterpret_cast<uint32_t*>(base + kNaOffPool);
+    SceneRec* rec = reinterpret_cast<SceneRec*>(base + kNaOffRec);
+    uint32_t* spill = reinterpret_cast<uint32_t*>(base + kNaOffSpill);
+    const uint32_t blocks = (n + 255) / 256;
+    k_pack3<<<blocks, 256, 0, c->stream>>>(o, ro, n);
+    k_pack3<<<blocks, 256, 0, c->stream>>>(d, rd, n);
+    SceneRec r{};
+    r.sc = s->view;
+    r.tmin = tmin;
+    k_noalloc_setup<<<1, kPools * kPoolStride, 0, c->stream>>>(pool, rec, r);
+    CK(c, hip_status(hipGetLastError(), "allocation-free query setup"));
+    ExtendArgs a{};
+    a.ray_o = ro;
+    a.ray_d = rd;
+    a.n = n;
+    a.pool = pool;
+    a.tmax = tmax;
+    a.hit = hit;
+    a.counts = counts;
+    a.spill = spill;
+    a.spill_stride = (uint32_t)kNoAllocGrid * kExtendBlock;
+    a.recs = rec;
+    a.one_scene = 1;
+    a.wide = 1;
+    CK(c, launch_extend(a, std::min(grid_for_extend(c, n), kNoAllocGrid), c->stream));
+    k_unpack_hits<<<blocks, 256, 0, c->stream>>>(hit, prim, t, n);
+    CK(c, hip_status(hipGetLastError(), "allocation-free query unpack"));
     return AURORA_OK;
 }
 

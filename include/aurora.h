@@ -307,6 +307,27 @@ int aurora_trace_closest_device_stats(aurora_scene* scene,
                                       float tmin, float tmax, int32_t* d_prim,
                                       float* d_t, int count_fetches,
                                       aurora_trace_stats* stats);
+/* Allocation-free form of aurora_trace_closest_device (same arguments and
+ * hits).  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no allocation, copy or
+ * synchronisation; every buffer beyond the caller's rays, hits and counts
+ * lives in a static device array of AURORA_NOALLOC_SCRATCH_BYTES (50 MiB)
+ * that each GPU the library runs on holds until the process ends (packed
+ * rays and hits, work counters, the scene table, the traversal stack
+ * spill).  At most AURORA_NOALLOC_MAX_RAYS rays per call (more:
+ * AURORA_ERR_INVALID); rays are traced in the given order (the
+ * aurora_set_ray_sort mode does not apply).  Asynchronous on the context
+ * stream.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other contexts or streams, other threads) race. */
+#define AURORA_NOALLOC_MAX_RAYS 262144u
+#define AURORA_NOALLOC_SCRATCH_BYTES 52430848u
+int aurora_trace_closest_device_noalloc(aurora_scene* scene,
+                                        const float* d_origins,
+                                        const float* d_dirs, uint32_t n,
+                                        float tmin, float tmax,
+                                        int32_t* d_prim, float* d_t,
+                                        uint32_t* d_counts);
 /* Brute-force closest hit (every primitive tested, no BVH): the semantics
  * of experiments/rt.cpp:209-218.  Parity aid. Device buffers. */
 int aurora_trace_brute_device(aurora_scene* scene, const float* d_origins,
