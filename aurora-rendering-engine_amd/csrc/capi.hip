@@ -24,9 +24,10 @@ using namespace aur;
 // automatic ray sorting of ray queries: at least this many rays on a tree of
 // at least this many nodes (32 bytes each: 64 MiB, beyond what the L2s hold)
 constexpr uint32_t kSortMinRays = 1u << 20;
-// resident 256-thread blocks of the fused tail kernel per CU (it is built
-// for five waves per SIMD)
+// resident 256-thread blocks of the fused tail kernel per CU: it is built for
+// five waves per SIMD for the tree walks, eight for the flat walk
 constexpr int kTailBlocksPerCu = 5;
+constexpr int kTailFlatBlocksPerCu = 8;
 constexpr uint32_t kSortMinNodes = 1u << 21;
 
 struct aurora_ctx {
@@ -530,7 +531,10 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
     }
     const int ext_grid = c->ext_grid;
     // the fused tail is persistent: its grid is the resident capacity
-    const int tail_grid = c->cus * kTailBlocksPerCu;
+    // the passes walk the flat leaf array under the condition render_jobs
+    // sets pa.flat by (one scene, small enough to have the array)
+    const bool flat = c->render_tree == 2 && jobs.size() == 1 && jobs[0].scene->view.n_flat > 0;
+    const int tail_grid = c->cus * (flat ? kTailFlatBlocksPerCu : kTailBlocksPerCu);
     const int graph_tail_grid = tail_grid;
     AUR_TRY(ensure_workspace(c, (uint32_t)cap, std::max(ext_grid, graph ? graph_tail_grid : tail_grid)));
     Workspace& w = c->ws;
@@ -619,7 +623,7 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         // the flat form needs one scene with a flat array; otherwise, and
         // in the wavefront traversal kernels, the four-wide tree stands in
         pa.wide = c->render_tree != 0 ? 1u : 0u;
-        pa.flat = c->render_tree == 2 && jobs.size() == 1 && jobs[0].scene->view.n_flat > 0 ? 1u : 0u;
+        pa.flat = flat ? 1u : 0u;
         pa.n_paths = path_off;
         pa.n_pixels = pix_off;
         same_shape = sig == prev_sig;

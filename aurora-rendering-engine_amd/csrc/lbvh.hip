@@ -22,6 +22,9 @@
 //
 // No reference counterpart: the reference scans every triangle for every ray
 // (experiments/rt.cpp:209-218).
+#include <algorithm>
+#include <vector>
+
 #include "internal.hpp"
 #include "traverse.hpp"
 
@@ -681,12 +684,33 @@ Status radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, 
     return Status::ok();
 }
 
+// The flat array holds the leaf records ordered by box surface area, largest
+// first (stable: Morton order among equal areas): a flat walk meets the big
+// occluders first, so its best t falls early and later boxes are culled
+// (1% on the Cornell box).  The order changes no hit.
 Status flat_build(uint32_t n, const DevBuf& nodes, const BvhScratch& s, DevBuf& flat, hipStream_t st) {
     if (n == 0 || n > kFlatMaxPrims) return Status::ok();
     AUR_TRY(flat.alloc((size_t)n * 32));
     k_flat_fill<<<blocks_for(n, kBlock), kBlock, 0, st>>>(n, s.leaf_pos.as<uint32_t>(), nodes.as<float4>(),
                                                           flat.as<float4>());
     AUR_HIP(hipGetLastError());
+    std::vector<float4> rec((size_t)2 * n), sorted((size_t)2 * n);
+    AUR_HIP(hipMemcpyAsync(rec.data(), flat.p, (size_t)n * 32, hipMemcpyDeviceToHost, st));
+    AUR_HIP(hipStreamSynchronize(st));
+    auto area = [&](uint32_t i) {
+        const float4 lo = rec[2 * (size_t)i], hi = rec[2 * (size_t)i + 1];
+        const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+        return dx * dy + dy * dz + dz * dx;
+    };
+    std::vector<uint32_t> idx(n);
+    for (uint32_t i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return area(a) > area(b); });
+    for (uint32_t i = 0; i < n; ++i) {
+        sorted[2 * (size_t)i] = rec[2 * (size_t)idx[i]];
+        sorted[2 * (size_t)i + 1] = rec[2 * (size_t)idx[i] + 1];
+    }
+    AUR_HIP(hipMemcpyAsync(flat.p, sorted.data(), (size_t)n * 32, hipMemcpyHostToDevice, st));
+    AUR_HIP(hipStreamSynchronize(st));
     return Status::ok();
 }
 

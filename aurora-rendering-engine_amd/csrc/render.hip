@@ -342,13 +342,15 @@ constexpr int kTailRefill = 4;
 // last claims (a 2 M-path stress render lost 5-15% at 128).
 constexpr uint32_t kTailBatchMin = 64, kTailBatchMax = 128;
 
-// five waves per SIMD: the register budget this leaves (84) holds the kernel
-// without spills; the resident grid (five blocks per CU) matches it.  TREE:
-// 1 traverse the four-wide tree, 0 the binary one, 2 (single-scene passes of
-// small scenes) the flat leaf array (the faster form depends on the scene,
-// so the launch shape chooses).
+// Five waves per SIMD for the tree walks: the register budget this leaves
+// holds them without spills; the resident grid (five blocks per CU) matches
+// it.  The flat walk needs no LDS stack and fewer registers, and runs eight
+// waves per SIMD (64 registers, a few spilled: 2.54 G paths/s at seven waves,
+// 2.58 at eight on the Cornell box).  TREE: 1 traverse the four-wide tree, 0
+// the binary one, 2 (single-scene passes of small scenes) the flat leaf array
+// (the faster form depends on the scene, so the launch shape chooses).
 template <bool ONE, int TREE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_tail(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 2 ? 8 : 5))) void k_tail(
     Pass pa, ShadeBufs b, uint32_t depth0, uint32_t* spill,
                                                  uint32_t spill_stride) {
     __shared__ uint32_t stk[kTravLds * kBlock];

@@ -167,9 +167,11 @@ __device__ __forceinline__ void traverse_closest_wide(const DevScene& sc, v3 o, 
     prim_out = miss ? AURORA_MISS : (int)best_id;
 }
 
-// Flat form, for scenes of a few primitives: every leaf record in Morton
-// order, its box's slab test against the current best t and its primitive
-// test, the same sequence for every ray.  Within a single-scene launch the
+// Flat form, for scenes of a few primitives: every leaf record in the flat
+// array's order (largest boxes first, so the best t falls early and more
+// boxes are culled), its box's slab test against the current best t and its
+// primitive test, the same sequence for every ray; a leaf whose box no lane
+// of the wave reaches is skipped at once.  Within a single-scene launch the
 // records sit at wave-uniform addresses (scalar loads) and no lane waits for
 // another's traversal: no stack, no divergence.  A primitive is taken under
 // the same conditions as in the tree walks (its leaf box passes the slab
@@ -193,6 +195,7 @@ __device__ __forceinline__ void traverse_closest_flat(const DevScene& sc, v3 o, 
         const uint32_t id = __float_as_uint(lo.w) & ~AURORA_NODE_LEAF;
         const bool valid = id < n_prims;
         const uint32_t q = valid ? id : 0u;
+        if (__ballot(hb) == 0) continue;  // no lane reaches this leaf's box: a wave-uniform skip
         float t = 0.0f;
         bool h;
         if (q < n_spheres) {

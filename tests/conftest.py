@@ -11,6 +11,16 @@ for p in (ROOT, PKG):
         sys.path.insert(0, p)
 
 
+def build_host_programs():
+    """Builds libare.so and the example programs through host/Makefile (a
+    no-op when they are up to date), one process at a time."""
+    import fcntl
+    import subprocess
+    with open(os.path.join(PKG, "host", ".build.lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        subprocess.check_call(["make", "-s", "-C", os.path.join(PKG, "host"), "all"])
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
 

@@ -1,0 +1,49 @@
+"""Writes tests/golden/cornell64_fp64_131072spp.npz: the Cornell box at
+64x64 (depth 50, Philox seed 0, background black) rendered by the CPU
+oracle in FP64 at 2^17 = 131072 samples per pixel (samples [0, 131072)),
+the "FP64 CPU render converged at 10^5 spp" of the north star at a size the
+host finishes in minutes.  Stored: the per-pixel FP64 radiance sums and the
+segment count.  GPU tests render the same samples in FP32 and compare.
+
+Run from the repository root:  python tests/golden/make_fp64_cornell.py
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+for p in (ROOT, os.path.join(ROOT, "aurora-rendering-engine_amd")):
+    sys.path.insert(0, p)
+
+from aurora_amd import scenes  # noqa: E402
+from aurora_amd.core import RenderSettings  # noqa: E402
+from oracle import orc  # noqa: E402
+
+SPP = 1 << 17
+CHUNK = 1 << 12
+
+
+def main(threads=int(os.environ.get("THREADS", "6"))):
+    cam = scenes.cornell_camera(64)
+    o = orc.OracleScene(scenes.cornell_box())
+    o.build_bvh()
+    total = np.zeros((64, 64, 3), np.float64)
+    segs = 0
+    t0 = time.time()
+    for off in range(0, SPP, CHUNK):
+        # chunked sample ranges summed in sample order per chunk; the chunk
+        # sums are added in double
+        rgb, _, _, s = o.render(cam, RenderSettings(spp=CHUNK, sample_offset=off, max_depth=50, seed=0), f64=True,
+                                threads=threads)
+        total += rgb
+        segs += s
+        print(f"{off + CHUNK} spp, {time.time() - t0:.0f} s", flush=True)
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cornell64_fp64_131072spp.npz")
+    np.savez_compressed(out, rgb_sum=total, spp=SPP, segments=segs, width=64, height=64, max_depth=50, seed=0)
+    print("wrote", out, flush=True)
+
+
+if __name__ == "__main__":
+    main()

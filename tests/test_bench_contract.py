@@ -1,6 +1,7 @@
 """bench.py's contract, checked on the CPU: a run on more than one GPU is
-refused with a message, and the table of measured HBM traffic covers the
-default Cornell step, so the default run reports roofline.traffic."""
+refused with a message, and the table of measured counters covers the
+default Cornell step, so the default run reports the roofline's issue
+fraction, lane utilization and HBM traffic."""
 import importlib.util
 import os
 import subprocess
@@ -23,10 +24,15 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU" in r.stderr
 
 
-def test_traffic_table_covers_default_step():
+def test_counter_table_covers_default_step():
     b = load_bench()
     # the default step: 1024^2, 32 spp, the whole pass in the persistent tail
-    # over the four-wide tree (the shape the autotuner picks on this scene)
-    key = ("k_tail", 1024, 32, b.TAIL_ALWAYS, True)
-    assert key in b.PMC_TRAFFIC and b.PMC_TRAFFIC[key] > 0
+    # over the flat leaf array (the shape the autotuner picks on this scene)
+    key = ("k_tail", 1024, 32, b.TAIL_ALWAYS, 2)
+    assert key in b.PMC and b.PMC[key]
+    row = b.PMC[key]
+    assert row["hbm_bytes"] > 0 and row["valu_insts"] > 0
+    # rocprof's VALUBusy reads up to half a percent above 100 on a saturated kernel
+    assert 0 < row["valu_util"] <= 100 and 0 < row["valu_busy"] <= 101
     assert b.HBM_PEAK_GBS == 8000.0
+    assert abs(b.VALU_PEAK_GINST - 614.4) < 1e-9

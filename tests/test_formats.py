@@ -24,7 +24,8 @@ TOOL = os.path.join(PKG, "progressive")
 
 @pytest.fixture(scope="module", autouse=True)
 def built():
-    subprocess.check_call(["make", "-s", "-C", os.path.join(PKG, "host"), "all"])
+    from conftest import build_host_programs
+    build_host_programs()
 
 
 def tool(*args):

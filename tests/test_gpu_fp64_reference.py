@@ -1,0 +1,45 @@
+"""The FP32 GPU render against the FP64 CPU oracle render converged at
+2^17 = 131072 spp (tests/golden/cornell64_fp64_131072spp.npz, written by
+tests/golden/make_fp64_cornell.py): the Cornell box at 64x64, depth 50,
+Philox seed 0, the same samples [0, 131072).
+
+Both sides draw the same random numbers; they differ only where FP32
+rounding sends a path another way than FP64 does.  Measured on MI355X: the
+FP32 paths end slightly sooner (0.08% fewer segments) and the image mean is
+0.1% lower, a systematic FP32 loss whose share of relMSE is at most 1e-6;
+the rest behaves like extra sampling noise and shrinks as 1/spp.  The bounds
+here, relMSE below 1e-5 (100x below the 1e-3 of the time-to-relMSE target),
+the mean within 0.2% and the segment count within 0.2%, are what make an
+FP32 GPU render of at least 2^17 spp a stand-in for the FP64 CPU judge."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from aurora_amd import scenes
+from aurora_amd.core import RenderSettings
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cornell64_fp64_131072spp.npz")
+
+
+def test_fp32_gpu_render_matches_fp64_oracle_at_2_17_spp(ctx):
+    g = np.load(GOLDEN)
+    spp = int(g["spp"])
+    ref = g["rgb_sum"] / spp
+    print("fp64 judge: rendering", spp, "spp at 64x64", flush=True)
+    sc = ctx.upload(scenes.cornell_box())
+    sc.build_bvh()
+    rgb = torch.zeros((64, 64, 3), dtype=torch.float32, device="cuda")
+    st = sc.render_device(scenes.cornell_camera(64), RenderSettings(spp=spp, max_depth=50, seed=0), rgb)
+    x = rgb.cpu().numpy().astype(np.float64) / spp
+    rel = float(np.mean((x - ref) ** 2 / (ref ** 2 + 0.01)))
+    seg_ratio = st["segments"] / float(g["segments"])
+    print(f"fp64 judge: relMSE(FP32 GPU, FP64 oracle) = {rel:.3e}, mean {x.mean():.6f} vs {ref.mean():.6f}, "
+          f"segments ratio {seg_ratio:.6f}", flush=True)
+    assert rel < 1e-5
+    assert abs(x.mean() - ref.mean()) < 2e-3 * ref.mean()
+    assert abs(seg_ratio - 1.0) < 2e-3
+    sc.close()

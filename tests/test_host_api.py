@@ -16,9 +16,8 @@ EXE = os.path.join(PKG, "cornell_example")
 
 @pytest.fixture(scope="module", autouse=True)
 def built():
-    """Builds libare.so and the example programs through host/Makefile (a
-    no-op when they are up to date)."""
-    subprocess.check_call(["make", "-s", "-C", os.path.join(PKG, "host"), "all"])
+    from conftest import build_host_programs
+    build_host_programs()
 
 
 def test_library_and_example_built():

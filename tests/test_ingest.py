@@ -25,7 +25,8 @@ REF_INCLUDE = "/root/reference/include"
 
 @pytest.fixture(scope="module", autouse=True)
 def built():
-    subprocess.check_call(["make", "-s", "-C", HOST, "all"])
+    from conftest import build_host_programs
+    build_host_programs()
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_INCLUDE), reason="reference headers not present")
