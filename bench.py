@@ -295,6 +295,29 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=65536):
     return out
 
 
+def heldout_section(ctx, seeds=(300000, 300001, 300002), width=1024, ref_spp=32768):
+    """Time to relMSE <= 1e-3 on Cornell variants (scenes.cornell_variant)
+    whose seeds no training stage used (training sets: seeds 0-255 at 256^2
+    and 200000-200127 at 1024^2; held-out checks during training: seeds from
+    100000), beside the canonical box: the
+    same plain and denoised pipelines as time_to_relmse_section, against a
+    reference of 2 x ref_spp samples per variant."""
+    from aurora_amd import scenes
+
+    out = []
+    for seed in seeds:
+        scene = ctx.upload(scenes.cornell_variant(seed))
+        scene.build_bvh()
+        cam = scenes.cornell_camera(width)
+        note(f"held-out variant {seed}")
+        r = time_to_relmse_section(ctx, scene, cam, ref_spp=ref_spp)
+        r["seed"] = seed
+        r.pop("denoised_curve", None)
+        out.append(r)
+        scene.close()
+    return out
+
+
 def denoiser_section(ctx, size=1024, batch=8, crop=128, reps=10):
     """Denoiser throughput: one inference on a size x size image (3x3 conv
     FLOPs of the 15 layers, 2*9*Cin*Cout per pixel at each level's
@@ -412,6 +435,7 @@ def serving_section(ctx, n_scenes=64, seed0=0, budget=SERVING_BUDGET, fallback_m
         else:
             pending.append(k)
     n_fallback = len(pending)
+    est0 = list(est)  # the denoised images' estimates
     fb_spp = list(halves)
     fb_paths = 0
     ctx.set_launch()
@@ -448,6 +472,8 @@ def serving_section(ctx, n_scenes=64, seed0=0, budget=SERVING_BUDGET, fallback_m
             "budget_relmse": budget, "denoised_kept": n_scenes - n_fallback, "fallbacks": n_fallback,
             "fallback_paths": fb_paths, "fallback_max_spp": fallback_max_spp,
             "met_budget": int(sum(e <= budget for e in est)),
+            "denoised_estimate_relmse": {"min": float(np.min(est0)), "median": float(np.median(est0)),
+                                         "max": float(np.max(est0))},
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "denoised": den is not None, "autotuned": {k: best[k] for k in best if k != "seconds"},
             "autotune_table": table, "setup_s": setup}
@@ -472,7 +498,8 @@ def main():
     ap.add_argument("--no-denoiser", action="store_true", help="skip the denoiser throughput section")
     ap.add_argument("--render-tree", choices=["binary", "wide", "flat"], default="binary",
                     help="render traversal tree when --tail-paths fixes the shape (no autotuning)")
-    ap.add_argument("--only", choices=["stress", "stress-render", "spheres", "copy", "serving", "denoiser"],
+    ap.add_argument("--only", choices=["stress", "stress-render", "spheres", "heldout", "copy", "serving",
+                                       "denoiser"],
                     default=None,
                     help="profiling modes: the stress section alone, or a plain 1 GiB device copy "
                          "(calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -506,6 +533,11 @@ def main():
     if args.only == "spheres":
         print(json.dumps(spheres_section(ctx)), flush=True)
         return
+    if args.only == "heldout":
+        ctx.set_launch(tail_paths=TAIL_ALWAYS)
+        ctx.set_render_tree(2)
+        print(json.dumps(heldout_section(ctx)), flush=True)
+        return
     if args.only == "serving":
         print(json.dumps(serving_section(ctx)), flush=True)
         return
@@ -530,7 +562,10 @@ def main():
         return scene.render_device(cam, rs, rgb, stats=stats or count)
 
     # launch shape for this workload from the autotuner (untimed; the shape
-    # changes speed only, never the image)
+    # changes speed only, never the image).  A step is one pass of three
+    # launches, so hipGraph capture gains nothing measurable (12.77 against
+    # 12.83 ms) and is left out: eager steps carry HIP events on every kernel,
+    # which the roofline reads from the timed region itself
     tune_table = []
     if args.tail_paths is None and not args.sweep_tail:
         from aurora_amd import autotune
@@ -544,7 +579,7 @@ def main():
         best, tune_table = autotune.tune(ctx, tune_run, f"cornell_{W}x{H}_spp{S}", cache_path=None, log=note,
                                          candidates={"extend_blocks_per_cu": [8], "shade_blocks_per_cu": [8],
                                                      "tail_paths": [1 << 20, TAIL_ALWAYS],
-                                                     "graph": [False, True], "tree": [0, 1, 2]})
+                                                     "graph": [False], "tree": [0, 1, 2]})
         graph = best["graph"]
         shape = {k: best[k] for k in best if k != "seconds"}
     else:
@@ -588,25 +623,29 @@ def main():
     # traversal rerun untimed with counting on (every segment in k_extend)
     cnt = step(args.warmup, count=True)
     alg_trav = 32 * cnt["nodes_fetched"] + 16 * cnt["spheres_fetched"] + 48 * cnt["quads_fetched"]
-    # kernel durations from HIP events on the library stream: captured passes
-    # carry no events, so that step is repeated eagerly (same kernels, same
-    # launch shape)
-    first = steps[0]
+    # kernel durations from HIP events on the library stream over the timed
+    # steps (eager launches; a shape forced to graph mode is repeated eagerly)
+    timed = steps
     if graph:
         graph = False
-        first = step(args.warmup)
+        timed = [step(args.warmup + i) for i in range(3)]
         graph = True
-    if first["tail_ms"] > first["extend_ms"]:
+    first = timed[0]
+    tail_ms_t = sum(s["tail_ms"] for s in timed)
+    ext_ms_t = sum(s["extend_ms"] for s in timed)
+    if tail_ms_t > ext_ms_t:
         # the fused tail carries the pass: its traversal bytes plus the path
         # queue read (48 B) and radiance write (16 B) of every path
-        kernel, launches, k_ms = "k_tail", max(1, first["tail_launches"]), first["tail_ms"]
-        alg_bytes = alg_trav + 64 * first["paths"]
+        kernel, k_ms = "k_tail", tail_ms_t
+        launches = max(1, sum(s["tail_launches"] for s in timed))
+        alg_bytes = (alg_trav + 64 * first["paths"]) * len(timed)
     else:
-        kernel, launches, k_ms = "k_extend", max(1, first["extend_launches"]), first["extend_ms"]
-        alg_bytes = alg_trav
+        kernel, k_ms = "k_extend", ext_ms_t
+        launches = max(1, sum(s["extend_launches"] for s in timed))
+        alg_bytes = alg_trav * len(timed)
     launch_s = k_ms / launches * 1e-3
     pmc = PMC.get((kernel, W, S, shape.get("tail_paths"), autotune_tree(shape)))
-    roof = {"kernel": kernel, "launches_per_step": launches, "avg_launch_ms": launch_s * 1e3,
+    roof = {"kernel": kernel, "launches_timed": launches, "avg_launch_ms": launch_s * 1e3,
             "cache_inclusive_GBs": alg_bytes / launches / launch_s / 1e9,
             "cache_inclusive_note": "node and primitive record bytes the traversal fetches (32 per node, 16 per "
                                     "sphere, 48 per quad, counted untimed by the counting kernel) plus 64 B of path "
