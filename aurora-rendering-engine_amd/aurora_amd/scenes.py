@@ -217,19 +217,21 @@ def random_spheres_camera(width=1200, height=675):
 
 def stress_scene(n_spheres=2_000_000, n_tris=2_000_000, seed=0):
     """Traversal stress test.  Draw order: sphere centres (3n), log10 radii
-    (n), triangle origins (3m), u directions (3m), u lengths (m), v
-    directions (3m), v lengths (m), material choice (n+m); directions are
-    normalised Gaussian-free cube samples (rejected if |d|<1e-3)."""
+    (n), triangle origins (3m), u directions (2m), u lengths (m), v
+    directions (2m), v lengths (m), material choice (n+m).  Edge directions
+    are uniform on the unit sphere: z = 2a - 1 and azimuth 2*pi*b from two
+    draws (a, b) (Archimedes: z of a uniform direction is uniform)."""
     rnd = Draws(seed)
     cen = rnd.array(3 * n_spheres).reshape(-1, 3)
     rad = 10.0 ** rnd.array(n_spheres, -3.0, -1.0)
     Q = rnd.array(3 * n_tris).reshape(-1, 3)
 
     def dirs(m):
-        d = rnd.array(3 * m, -1.0, 1.0).reshape(-1, 3)
-        l = np.linalg.norm(d, axis=1, keepdims=True)
-        d = np.where(l < 1e-3, np.array([[1.0, 0.0, 0.0]]), d)
-        return d / np.linalg.norm(d, axis=1, keepdims=True)
+        ab = rnd.array(2 * m).reshape(-1, 2)
+        z = 2.0 * ab[:, 0] - 1.0
+        phi = 2.0 * np.pi * ab[:, 1]
+        s = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+        return np.stack([s * np.cos(phi), s * np.sin(phi), z], axis=1)
 
     ud = dirs(n_tris)
     ul = rnd.array(n_tris, 0.001, 0.05)
