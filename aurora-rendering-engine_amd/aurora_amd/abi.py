@@ -6,7 +6,7 @@ sizes against the C compiler's (tests/test_abi.py).
 import ctypes as C
 import os
 
-ABI_VERSION = 3  # AURORA_ABI_VERSION of the header this mirror follows
+ABI_VERSION = 4  # AURORA_ABI_VERSION of the header this mirror follows
 
 AURORA_OK = 0
 AURORA_ERR_INVALID = 1
@@ -193,6 +193,8 @@ EXPORTS = [
     ("aurora_camera_rays_device", C.c_int, [_vp, C.POINTER(CameraDesc), _u32, _vp, _vp, _u32, _vp, _vp]),
     ("aurora_paste", C.c_int, [_vp, _vp, _u32, _u32, _vp, _u32, _u32, _vp]),
     ("aurora_paste_device", C.c_int, [_vp, _vp, _u32, _u32, _vp, _u32, _u32, _vp]),
+    ("aurora_patch_render", C.c_int, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
+    ("aurora_patch_render_device", C.c_int, [_vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp]),
     ("aurora_resolve_rgb8", C.c_int, [_vp, _u32, _u32, _u32, _u32, _vp]),
     ("aurora_resolve_rgb8_device", C.c_int, [_vp, _vp, _u32, _u32, _u32, _u32, _vp]),
     ("aurora_render_batch_device", C.c_int, [_vp, C.POINTER(BatchItem), _u32, _u32, C.POINTER(RenderStats)]),

@@ -240,6 +240,33 @@ class Context:
         self.check(self.lib.aurora_paste_device(self.h, src.data_ptr(), src.shape[1], src.shape[0], dst.data_ptr(),
                                                 dst.shape[1], dst.shape[0], c.ctypes.data))
 
+    def patch_render(self, tris, mats, cam, cfg):
+        """experiments/rt10.cpp's patch-projection renderer on the GPU
+        (aurora_patch_render): tris, mats, cam, cfg are aurora_amd.patch
+        structured arrays; returns (image (h, w, 3) float64, stats dict)."""
+        from . import patch
+
+        t, m, c, g = (np.ascontiguousarray(x) for x in (tris, mats, cam, cfg))
+        assert t.dtype == patch.TRIANGLE and m.dtype == patch.MATERIAL
+        assert c.dtype == patch.CAMERA and g.dtype == patch.CONFIG
+        img = np.zeros((int(c["height"].item()), int(c["width"].item()), 3), np.float64)
+        st = patch.Stats()
+        self.check(self.lib.aurora_patch_render(self.h, t.ctypes.data if t.size else None, t.size,
+                                                m.ctypes.data if m.size else None, m.size, c.ctypes.data,
+                                                g.ctypes.data, img.ctypes.data, C.byref(st)))
+        return img, {k: getattr(st, k) for k, _ in st._fields_}
+
+    def patch_render_device(self, tris, mats, cam, cfg, out):
+        """Same into a float64 device tensor out (h, w, 3); returns stats."""
+        from . import patch
+
+        t, m, c, g = (np.ascontiguousarray(x) for x in (tris, mats, cam, cfg))
+        st = patch.Stats()
+        self.check(self.lib.aurora_patch_render_device(self.h, t.ctypes.data if t.size else None, t.size,
+                                                       m.ctypes.data if m.size else None, m.size, c.ctypes.data,
+                                                       g.ctypes.data, out.data_ptr(), C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
     def resolve_rgb8_device(self, rgb_sum, spp, out, transfer=abi.TRANSFER_GAMMA22):
         """8-bit resolve of a device radiance sum (H, W, 3) float32 into a
         uint8 tensor of the same shape.  Asynchronous on the context stream."""

@@ -50,6 +50,7 @@ struct aurora_ctx {
     int recs_turn = 0;
     uint32_t host_recs_cap = 0;
     unsigned char* scratch = nullptr;  // this device's copy of the allocation-free scratch array
+    PatchWork patch;                   // patch-projection renderer tables
 };
 
 struct aurora_scene {
@@ -881,6 +882,7 @@ void aurora_destroy(aurora_ctx* c) {
     DevBuf* bufs[] = {&w.q_o[0], &w.q_o[1], &w.q_d[0], &w.q_d[1], &w.q_b[0], &w.q_b[1], &w.hit, &w.L,
                       &w.aov_a, &w.aov_n, &w.counters, &w.spill, &w.recs};
     for (DevBuf* b : bufs) b->release();
+    c->patch.release();
     if (w.host_counts) (void)hipHostFree(w.host_counts);
     for (int k = 0; k < 2; ++k) {
         if (c->host_recs[k]) (void)hipHostFree(c->host_recs[k]);
@@ -1317,6 +1319,32 @@ int aurora_paste(aurora_ctx* c, const float* src, uint32_t sw, uint32_t sh, floa
     if (!r.bad()) r = hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     (void)hipStreamSynchronize(c->stream);
     if (r.bad()) return fail(c, r);
+    return AURORA_OK;
+}
+
+int aurora_patch_render_device(aurora_ctx* c, const aurora_patch_triangle* tris, uint32_t n_tris,
+                               const aurora_patch_material* mats, uint32_t n_mats, const aurora_patch_camera* cam,
+                               const aurora_patch_config* cfg, double* d_rgb, aurora_patch_stats* stats) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (!d_rgb) return fail(c, AURORA_ERR_INVALID, "patch render: output is NULL");
+    CK(c, patch_check(tris, n_tris, mats, n_mats, cam, cfg));
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    CK(c, patch_render_device(c->patch, tris, n_tris, mats, cam, cfg, d_rgb, c->stream, stats));
+    return AURORA_OK;
+}
+
+int aurora_patch_render(aurora_ctx* c, const aurora_patch_triangle* tris, uint32_t n_tris,
+                        const aurora_patch_material* mats, uint32_t n_mats, const aurora_patch_camera* cam,
+                        const aurora_patch_config* cfg, double* rgb, aurora_patch_stats* stats) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (!rgb) return fail(c, AURORA_ERR_INVALID, "patch render: output is NULL");
+    CK(c, patch_check(tris, n_tris, mats, n_mats, cam, cfg));
+    CK(c, hip_status(hipSetDevice(c->device), "hipSetDevice"));
+    const size_t bytes = size_t(cam->width) * cam->height * 3 * sizeof(double);
+    CK(c, c->patch.out.alloc(bytes));
+    CK(c, patch_render_device(c->patch, tris, n_tris, mats, cam, cfg, c->patch.out.as<double>(), c->stream, stats));
+    CK(c, hip_status(hipMemcpyAsync(rgb, c->patch.out.p, bytes, hipMemcpyDeviceToHost, c->stream), "copy"));
+    CK(c, hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize"));
     return AURORA_OK;
 }
 

@@ -198,6 +198,25 @@ Status launch_camera_rays(const CamF& cam, uint32_t seed, const uint32_t* pixels
 // false when there is nothing to paste (the reference's early returns)
 Status paste_device(const float* src, uint32_t sw, uint32_t sh, float* dst, uint32_t dw, uint32_t dh,
                     const int32_t* corners, hipStream_t s, bool* painted);
+
+// Patch-projection renderer (experiments/rt10.cpp): device tables of one
+// render (texture records, warp triangles, tiles, the texel pool of the
+// recursed textures) and the host-output staging image, kept by the context
+// and grown as needed.
+struct PatchWork {
+    DevBuf texs, warps, tiles, pool, out;
+    void release() {
+        DevBuf* all[] = {&texs, &warps, &tiles, &pool, &out};
+        for (DevBuf* b : all) b->release();
+    }
+};
+Status patch_check(const aurora_patch_triangle* tris, uint32_t n_tris, const aurora_patch_material* mats,
+                   uint32_t n_mats, const aurora_patch_camera* cam, const aurora_patch_config* cfg);
+// plan on the host, rasterise on stream s into d_out (H x W x 3 doubles);
+// returns after the raster launches completed (stats->kernel_ms)
+Status patch_render_device(PatchWork& pw, const aurora_patch_triangle* tris, uint32_t n_tris,
+                           const aurora_patch_material* mats, const aurora_patch_camera* cam,
+                           const aurora_patch_config* cfg, double* d_out, hipStream_t s, aurora_patch_stats* stats);
 // 8-bit resolve of n floats (sum and out 16-byte and 4-byte aligned)
 Status launch_resolve_rgb8(const float* sum, size_t n, uint32_t spp, uint32_t transfer, uint8_t* out, hipStream_t s);
 Status launch_shade(const Pass& p, Workspace& ws, int cur, uint32_t depth, int grid, hipStream_t s);

@@ -479,6 +479,83 @@ def serving_section(ctx, n_scenes=64, seed0=0, budget=SERVING_BUDGET, fallback_m
             "autotune_table": table, "setup_s": setup}
 
 
+def patch_section(ctx, reps=20, cpu_seconds=6.0):
+    """The reference's patch-projection renderer (experiments/rt10.cpp) on
+    the GPU: its own program's scene at its configuration (900x650, depth 4,
+    textures 16..256), and a corridor of two facing mirrors at 1920x1080
+    (recursion to depth 4, textures up to 512).  A frame is one
+    aurora_patch_render_device call: the host plans the recursion tree, the
+    device rasterises every texture and the image into a device buffer.
+    The reference itself (rt10.cpp compiled from its source, oracle/_ref),
+    or the oracle where it was not built, renders the same frames on one
+    host core; every GPU image is checked bit for bit against the oracle."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from aurora_amd import patch
+    from oracle import orc
+
+    ref_lib = os.path.join(ROOT, "oracle", "_ref", "librt10.so")
+    ref = C.CDLL(ref_lib) if os.path.exists(ref_lib) else None
+
+    def cpu_frame(t, m, c, g):
+        if ref is None:
+            return orc.patch_render(t, m, c, g)[0]
+        img = np.zeros((int(c["height"].item()), int(c["width"].item()), 3))
+        vp = C.c_void_p
+        rc = ref.ref_rt10_render(vp(t.ctypes.data), C.c_uint32(t.size), vp(m.ctypes.data), C.c_uint32(m.size),
+                                 vp(c.ctypes.data), vp(g.ctypes.data), vp(img.ctypes.data))
+        assert rc == 0
+        return img
+
+    out = {}
+    for name, case in (("rt10_cornell_900x650", patch.rt10_cornell()),
+                       ("mirror_hall_1920x1080", patch.mirror_hall(tiles=6, width=1920, height=1080, max_depth=4,
+                                                                  max_tex_res=512))):
+        t, m, c, g = case
+        W, H = int(c["width"].item()), int(c["height"].item())
+        dev = torch.empty((H, W, 3), dtype=torch.float64, device="cuda")
+        st = ctx.patch_render_device(t, m, c, g, dev)
+        same = bool(np.array_equal(dev.cpu().numpy().view(np.uint64), orc.patch_render(t, m, c, g)[0].view(np.uint64)))
+        torch.cuda.synchronize()
+        sts = []
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sts.append(ctx.patch_render_device(t, m, c, g, dev))
+        torch.cuda.synchronize()
+        frame_s = (time.perf_counter() - t0) / reps
+        plan_ms = sum(x["plan_ms"] for x in sts) / reps
+        kern_ms = sum(x["kernel_ms"] for x in sts) / reps
+        t0 = time.perf_counter()
+        host_img, _ = ctx.patch_render(t, m, c, g)
+        host_s = time.perf_counter() - t0
+        n_cpu, cpu_s = 0, 0.0
+        while cpu_s < cpu_seconds and n_cpu < 50:
+            t0 = time.perf_counter()
+            cpu_frame(t, m, c, g)
+            cpu_s += time.perf_counter() - t0
+            n_cpu += 1
+        cpu_frame_s = cpu_s / n_cpu
+        out[name] = {
+            "width": W, "height": H, "triangles": int(t.size), "textures": st["textures"],
+            "raster_textures": st["raster_textures"], "levels": st["levels"], "warps": st["warps"],
+            "texels": st["texels"], "warp_texels": st["warp_texels"],
+            "gpu_frame_ms": frame_s * 1e3, "gpu_frames_per_s": 1.0 / frame_s,
+            "plan_ms": plan_ms, "raster_kernel_ms": kern_ms,
+            "raster_warp_texels_per_s": st["warp_texels"] / (kern_ms * 1e-3) if kern_ms > 0 else None,
+            "host_output_frame_ms": host_s * 1e3,
+            "cpu_frame_ms": cpu_frame_s * 1e3, "cpu_kind": "reference" if ref is not None else "port",
+            "cpu_frames": n_cpu, "cpu_cores": 1,
+            "speedup_vs_cpu": cpu_frame_s / frame_s,
+            "bit_equal_to_oracle": same,
+        }
+        note(f"patch {name}: {frame_s * 1e3:.2f} ms/frame on the GPU (plan {plan_ms:.2f} ms, raster "
+             f"{kern_ms:.3f} ms), {cpu_frame_s * 1e3:.1f} ms on one host core; bit-equal {same}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -498,8 +575,9 @@ def main():
     ap.add_argument("--no-denoiser", action="store_true", help="skip the denoiser throughput section")
     ap.add_argument("--render-tree", choices=["binary", "wide", "flat"], default="binary",
                     help="render traversal tree when --tail-paths fixes the shape (no autotuning)")
+    ap.add_argument("--no-patch", action="store_true", help="skip the patch-projection renderer section")
     ap.add_argument("--only", choices=["stress", "stress-render", "spheres", "heldout", "copy", "serving",
-                                       "denoiser"],
+                                       "denoiser", "patch"],
                     default=None,
                     help="profiling modes: the stress section alone, or a plain 1 GiB device copy "
                          "(calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -543,6 +621,9 @@ def main():
         return
     if args.only == "denoiser":
         print(json.dumps(denoiser_section(ctx)), flush=True)
+        return
+    if args.only == "patch":
+        print(json.dumps(patch_section(ctx)), flush=True)
         return
     if args.tail_paths is not None:
         ctx.set_launch(tail_paths=args.tail_paths)
@@ -701,6 +782,8 @@ def main():
         ctx.set_launch()
         out["serving"] = serving_section(ctx)
         ctx.set_launch()
+    if not args.no_patch:
+        out["patch_projection"] = patch_section(ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(arr, cam, S)
     print(json.dumps(out), flush=True)

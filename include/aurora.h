@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define AURORA_ABI_VERSION 3
+#define AURORA_ABI_VERSION 4
 
 /* ---- status codes ---------------------------------------------------- */
 #define AURORA_OK 0
@@ -432,6 +432,94 @@ int aurora_paste_device(aurora_ctx* ctx, const float* d_src, uint32_t src_w,
 int aurora_paste(aurora_ctx* ctx, const float* src, uint32_t src_w,
                  uint32_t src_h, float* dst, uint32_t dst_w, uint32_t dst_h,
                  const int32_t* corners);
+
+/* ---- patch-projection renderer ------------------------------------------
+ * The recursive mirror renderer of experiments/rt10.cpp:551-772, all in
+ * double as there.  Each viewport triangle paints, far to near (triangle
+ * centroid distance from the eye), every scene triangle with a vertex that
+ * projects inside it, as that triangle's texture warped through the
+ * Sutherland-Hodgman clip of its projection; a reflective triangle's
+ * texture is rendered by the same procedure from the eye mirrored across
+ * its plane, onto the triangle itself, and mixed with its albedo.  The
+ * recursion stops at max_depth, below min_area_px estimated texels, and at
+ * a triangle id already on the recursion path.  The recursion tree and the
+ * clipped warp triangles are planned on the host; every texel of every
+ * texture and of the image is rasterised on the device, one launch per
+ * recursion level, deepest first. */
+#define AURORA_PATCH_DIFFUSE 0u
+#define AURORA_PATCH_REFLECTIVE 1u
+#define AURORA_PATCH_NO_MATERIAL 0xffffffffu  /* white diffuse (rt10.cpp:570-574) */
+#define AURORA_PATCH_MAX_DEPTH 64
+#define AURORA_PATCH_MAX_TEX_RES 4096
+
+/* rt10.cpp:145-155 MaterialType, Material */
+typedef struct aurora_patch_material {
+    uint32_t type;         /* AURORA_PATCH_DIFFUSE / AURORA_PATCH_REFLECTIVE */
+    uint32_t reserved;     /* 0                                            */
+    double albedo[3];
+    double metalness;      /* reflective: weight of the albedo-tinted
+                              reflection, clamped to [0, 1]                */
+} aurora_patch_material;
+
+/* rt10.cpp:157-169 Triangle */
+typedef struct aurora_patch_triangle {
+    double p[3][3];        /* vertices                                     */
+    double uv[3][2];       /* texture coordinates, v down                  */
+    int32_t id;            /* recursion-guard identity                     */
+    uint32_t material;     /* material index or AURORA_PATCH_NO_MATERIAL   */
+} aurora_patch_triangle;
+
+/* rt10.cpp:670-676 Camera: an eye and two viewport triangles whose uv
+ * (in [0,1]^2, v down) place them in the image */
+typedef struct aurora_patch_camera {
+    double origin[3];
+    aurora_patch_triangle viewport[2];  /* id and material unused        */
+    uint32_t width, height;             /* >= 2                          */
+} aurora_patch_camera;
+
+/* rt10.cpp:543-549 RenderConfig */
+typedef struct aurora_patch_config {
+    int32_t max_depth;     /* 0 .. AURORA_PATCH_MAX_DEPTH                  */
+    int32_t max_tex_res;   /* texture side clamp, 1 <= min <= max <=      */
+    int32_t min_tex_res;   /*   AURORA_PATCH_MAX_TEX_RES                   */
+    uint32_t reserved;     /* 0                                            */
+    double min_area_px;    /* no recursion below this many texels          */
+    double env[3];         /* colour where a reflection sees nothing       */
+} aurora_patch_config;
+
+typedef struct aurora_patch_stats {
+    uint32_t textures;         /* triangle textures in the recursion tree  */
+    uint32_t raster_textures;  /* of them rasterised (reflective and
+                                  recursed; the others are constant)       */
+    uint32_t levels;           /* recursion levels with rasterised textures */
+    uint32_t warps;            /* warp triangles (clipped-polygon fans)     */
+    uint64_t texels;           /* texels rasterised, viewports included     */
+    uint64_t warp_texels;      /* texel x warp-triangle pairs in the warp's
+                                  clamped pixel box (the reference's
+                                  per-pixel loop iterations)                */
+    double plan_ms;            /* host: visibility, clipping, recursion     */
+    double kernel_ms;          /* device: the raster launches               */
+} aurora_patch_stats;
+
+/* Render into rgb (height x width x 3 doubles, row 0 at the top, clamped
+ * to [0,1] as rt10.cpp:775-784).  Host buffer; synchronous.  The image is
+ * the reference's bit for bit.  Errors: AURORA_ERR_INVALID for bad sizes,
+ * material indices or configuration; AURORA_ERR_NOMEM when the recursion
+ * tree's textures exceed the device (stats then hold the plan's size).
+ * stats may be NULL. */
+int aurora_patch_render(aurora_ctx* ctx, const aurora_patch_triangle* tris,
+                        uint32_t n_tris, const aurora_patch_material* mats,
+                        uint32_t n_mats, const aurora_patch_camera* cam,
+                        const aurora_patch_config* cfg, double* rgb,
+                        aurora_patch_stats* stats);
+/* Same with a device output buffer (waits for the raster launches). */
+int aurora_patch_render_device(aurora_ctx* ctx,
+                               const aurora_patch_triangle* tris,
+                               uint32_t n_tris,
+                               const aurora_patch_material* mats,
+                               uint32_t n_mats, const aurora_patch_camera* cam,
+                               const aurora_patch_config* cfg, double* d_rgb,
+                               aurora_patch_stats* stats);
 
 /* ---- learned denoiser ---------------------------------------------------
  * A 4-level U-Net (32/48/64/64 channels, 15 3x3 convolutions) on BF16 MFMA

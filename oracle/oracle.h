@@ -77,6 +77,13 @@ int orc_resolve_rgb8(const float* sum, uint64_t n, uint32_t spp, uint32_t transf
 int orc_render_pixels(orc_scene* s, const aurora_camera_desc* c, const aurora_render_desc* rd,
                       const uint32_t* pixels, uint32_t n, int threads, float* rgb, float* albedo, float* normal,
                       uint64_t* segments);
+/* experiments/rt10.cpp's patch-projection renderer restated in double:
+ * rgb receives height x width x 3 (row 0 at the top); textures (optional)
+ * the number of triangle textures of the recursion tree */
+int orc_patch_render(const aurora_patch_triangle* tris, uint32_t n_tris, const aurora_patch_material* mats,
+                     uint32_t n_mats, const aurora_patch_camera* cam, const aurora_patch_config* cfg, double* rgb,
+                     uint64_t* textures);
+const char* orc_patch_last_error(void);
 int orc_render_pixels_f64(orc_scene* s, const aurora_camera_desc* c, const aurora_render_desc* rd,
                           const uint32_t* pixels, uint32_t n, int threads, double* rgb, double* albedo,
                           double* normal, uint64_t* segments);

@@ -55,6 +55,8 @@ def lib():
             "orc_camera_rays": (C.c_int, [C.POINTER(abi.CameraDesc), u32, vp, vp, u32, vp, vp]),
             "orc_resolve_rgb8": (C.c_int, [vp, C.c_uint64, u32, u32, vp]),
             "orc_paste": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, vp]),
+            "orc_patch_render": (C.c_int, [vp, C.c_uint32, vp, C.c_uint32, vp, vp, vp, vp]),
+            "orc_patch_last_error": (C.c_char_p, []),
             "orc_render_pixels": (C.c_int, [vp, C.POINTER(abi.CameraDesc), C.POINTER(abi.RenderDesc), vp, u32,
                                             C.c_int, vp, vp, vp, vp]),
             "orc_render_pixels_f64": (C.c_int, [vp, C.POINTER(abi.CameraDesc), C.POINTER(abi.RenderDesc), vp,
@@ -175,6 +177,23 @@ def paste(src, dst, corners):
     c = np.ascontiguousarray(corners, np.int32).reshape(8)
     n = lib().orc_paste(s.ctypes.data, s.shape[1], s.shape[0], d.ctypes.data, d.shape[1], d.shape[0], c.ctypes.data)
     return d, n
+
+
+def patch_render(tris, mats, cam, cfg):
+    """experiments/rt10.cpp's patch-projection renderer restated: returns
+    (image (h, w, 3) float64, triangle textures of the recursion tree).
+    Arguments are aurora_amd.patch structured arrays."""
+    t = np.ascontiguousarray(tris)
+    m = np.ascontiguousarray(mats)
+    c = np.ascontiguousarray(cam)
+    g = np.ascontiguousarray(cfg)
+    img = np.zeros((int(c["height"].item()), int(c["width"].item()), 3), np.float64)
+    n = C.c_uint64(0)
+    rc = lib().orc_patch_render(t.ctypes.data if t.size else None, t.size, m.ctypes.data if m.size else None, m.size,
+                                c.ctypes.data, g.ctypes.data, img.ctypes.data, C.byref(n))
+    if rc != 0:
+        raise ValueError(lib().orc_patch_last_error().decode())
+    return img, n.value
 
 
 def hit_sphere(sphere, o, d, tmin=0.0, tmax=np.inf):

@@ -75,3 +75,33 @@ def test_create_without_device_reports_error():
     rc = lib.aurora_create(0, C.byref(h))
     assert rc != abi.AURORA_OK
     assert len(lib.aurora_last_error(None)) > 0
+
+
+def test_patch_descriptor_layouts_match_c():
+    from aurora_amd import patch
+
+    src = """
+#include <stdio.h>
+#include <stddef.h>
+#include "aurora.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\\n", sizeof(aurora_patch_material), sizeof(aurora_patch_triangle),
+    sizeof(aurora_patch_camera), sizeof(aurora_patch_config), sizeof(aurora_patch_stats));
+  printf("%zu %zu %zu %zu %zu %zu %zu\\n", offsetof(aurora_patch_material, albedo),
+    offsetof(aurora_patch_triangle, uv), offsetof(aurora_patch_triangle, id),
+    offsetof(aurora_patch_camera, viewport), offsetof(aurora_patch_camera, width),
+    offsetof(aurora_patch_config, min_area_px), offsetof(aurora_patch_stats, plan_ms));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "s.c")
+        open(p, "w").write(src)
+        exe = os.path.join(d, "s")
+        subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), p, "-o", exe])
+        got = [int(x) for x in subprocess.check_output([exe]).decode().split()]
+    exp = [patch.MATERIAL.itemsize, patch.TRIANGLE.itemsize, patch.CAMERA.itemsize, patch.CONFIG.itemsize,
+           C.sizeof(patch.Stats), patch.MATERIAL.fields["albedo"][1], patch.TRIANGLE.fields["uv"][1],
+           patch.TRIANGLE.fields["id"][1], patch.CAMERA.fields["viewport"][1], patch.CAMERA.fields["width"][1],
+           patch.CONFIG.fields["min_area_px"][1], patch.Stats.plan_ms.offset]
+    assert got == exp
