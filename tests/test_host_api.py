@@ -56,3 +56,25 @@ def test_cpp_render_matches_python_path(tmp_path, ctx):
     # the ray query of the example hits the same primitive as the Python trace
     prim, t = g.trace(np.array([[278, 278, -800]], np.float32), np.array([[0, 0, 1]], np.float32), 0.001)
     assert f"hit prim {prim[0]} " in r.stdout
+
+
+RT10 = os.path.join(PKG, "rt10_scene")
+
+
+def test_patch_api_example_built():
+    # are::patch (host/include/are/patch.hpp) and the example that builds
+    # experiments/rt10.cpp's scene with it
+    assert os.path.exists(RT10)
+
+
+@pytest.mark.gpu
+def test_patch_api_writes_the_reference_programs_file(tmp_path):
+    import zlib
+
+    out = tmp_path / "output_rt10.ppm"
+    r = subprocess.run([RT10, str(out), "--stats"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    golden = zlib.decompress(open(os.path.join(ROOT, "tests", "golden", "rt10_cornell_900x650.ppm.zlib"), "rb").read())
+    assert out.read_bytes() == golden
+    assert "textures" in r.stdout
