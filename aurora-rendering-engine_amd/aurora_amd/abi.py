@@ -6,7 +6,7 @@ sizes against the C compiler's (tests/test_abi.py).
 import ctypes as C
 import os
 
-ABI_VERSION = 4  # AURORA_ABI_VERSION of the header this mirror follows
+ABI_VERSION = 5  # AURORA_ABI_VERSION of the header this mirror follows
 
 AURORA_OK = 0
 AURORA_ERR_INVALID = 1
@@ -173,6 +173,7 @@ EXPORTS = [
     ("aurora_set_launch", C.c_int, [_vp, C.c_int, C.c_int, _u32]),
     ("aurora_set_ray_sort", C.c_int, [_vp, C.c_int]),
     ("aurora_set_render_tree", C.c_int, [_vp, C.c_int]),
+    ("aurora_set_queue", C.c_int, [_vp, _u32, _u32]),
     ("aurora_scene_upload", C.c_int, [_vp, C.POINTER(SceneDesc), C.POINTER(_vp)]),
     ("aurora_scene_destroy", None, [_vp]),
     ("aurora_bvh_build", C.c_int, [_vp, C.POINTER(BvhStats)]),

@@ -5,8 +5,10 @@ shape, and the fastest is kept and cached under a signature of the
 workload (pixels, samples, primitives).  The shape covers the persistent
 traversal and shading grids (blocks per compute unit), the live-path count
 at which the fused tail takes over, whether passes are captured in a
-hipGraph, and which tree (binary, four-wide, or the flat leaf array of a
-small scene) the render kernels walk.
+hipGraph, which tree (binary, four-wide, or the flat leaf array of a small
+scene) the render kernels walk, and the queue sizes: the most paths per
+pass and the paths a wave claims per work-pool atomic (candidates add them
+as "pass_paths" / "claim_paths"; 0 is the library's choice).
 Shapes change speed only, never results (tests check the images bit for
 bit), so the tuner can be run at any time.
 """
@@ -38,6 +40,8 @@ def apply(ctx, shape):
     ctx.set_launch(shape["extend_blocks_per_cu"], shape["shade_blocks_per_cu"], shape["tail_paths"])
     if hasattr(ctx, "set_render_tree"):
         ctx.set_render_tree(tree_of(shape))
+    if hasattr(ctx, "set_queue"):
+        ctx.set_queue(shape.get("pass_paths", 0), shape.get("claim_paths", 0))
 
 
 def signature(items):
@@ -86,3 +90,15 @@ def tune(ctx, run, sig, cache_path=DEFAULT_CACHE, candidates=None, repeats=2, lo
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1)
     return best, table
+
+
+QUEUE_CANDIDATES = {"pass_paths": [0, 1 << 25, 1 << 26], "claim_paths": [0, 32, 256]}
+
+
+def refine(ctx, run, sig, best, candidates=None, cache_path=DEFAULT_CACHE, repeats=2, log=None):
+    """Second stage: with the launch shape `best` fixed, time the queue sizes
+    (most paths per pass, paths per work-pool claim; the library's choice,
+    0, among them) and keep the fastest.  Returns (best shape, table)."""
+    cand = {k: [v] for k, v in best.items() if k != "seconds"}
+    cand.update(candidates or QUEUE_CANDIDATES)
+    return tune(ctx, run, sig + "_queue", cache_path=cache_path, candidates=cand, repeats=repeats, log=log)

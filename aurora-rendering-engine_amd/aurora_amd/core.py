@@ -187,6 +187,11 @@ class Context:
         the images are the same."""
         self.check(self.lib.aurora_set_render_tree(self.h, int(tree)))
 
+    def set_queue(self, pass_paths=0, claim_paths=0):
+        """Queue sizes: most paths per automatic pass (0: 2^27) and paths per
+        work-pool claim in the fused tail (0: by pass size).  Speed only."""
+        self.check(self.lib.aurora_set_queue(self.h, int(pass_paths), int(claim_paths)))
+
     def set_ray_sort(self, mode=1):
         """Ray queries: 0 trace in the given order, 1 sort large queries on
         large trees (default), 2 always sort.  Hits are the same."""

@@ -41,6 +41,8 @@ struct aurora_ctx {
     uint32_t graph_tail_depth = 8;   // captured passes switch to the fused tail here
     int ray_sort = 1;                // ray queries: 0 never sort, 1 large scenes, 2 always
     uint32_t render_tree = 0;        // render bounces: 0 binary LBVH, 1 four-wide tree, 2 flat (small scenes)
+    uint64_t pass_paths = 0;         // automatic passes: at most this many paths (0: kPassPaths)
+    uint32_t claim_paths = 0;        // fused-tail paths per pool claim (0: by pass size)
     Workspace ws;                    // path queues shared by the context's renders
     // pinned staging of the passes' scene tables, two in turn: a table is
     // rewritten only after the copy that read it two passes ago has run
@@ -496,7 +498,8 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         if (j.rd.spp_per_pass) fixed += std::min<uint64_t>(j.rd.spp_per_pass, std::max<uint32_t>(1, j.rd.spp)) * wh;
         else max_spp = std::max(max_spp, j.rd.spp);
     }
-    const uint64_t room = kPassPaths > fixed ? kPassPaths - fixed : 0;
+    const uint64_t pass_cap = c->pass_paths ? c->pass_paths : kPassPaths;
+    const uint64_t room = pass_cap > fixed ? pass_cap - fixed : 0;
     auto auto_paths = [&](uint64_t P) {
         uint64_t n = 0;
         for (const Job& j : jobs)
@@ -625,6 +628,7 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         // in the wavefront traversal kernels, the four-wide tree stands in
         pa.wide = c->render_tree != 0 ? 1u : 0u;
         pa.flat = flat ? 1u : 0u;
+        pa.claim = c->claim_paths;
         pa.n_paths = path_off;
         pa.n_pixels = pix_off;
         same_shape = sig == prev_sig;
@@ -909,6 +913,17 @@ int aurora_set_launch(aurora_ctx* c, int ext_blocks_per_cu, int shade_blocks_per
     c->ext_grid = c->cus * ext_blocks_per_cu;
     c->shade_grid = c->cus * shade_blocks_per_cu;
     c->tail_paths = tail_paths;
+    return AURORA_OK;
+}
+
+int aurora_set_queue(aurora_ctx* c, uint32_t pass_paths, uint32_t claim_paths) {
+    if (!c) return fail(nullptr, AURORA_ERR_INVALID, "ctx is NULL");
+    if (pass_paths && (pass_paths < (1u << 16) || pass_paths > (1u << 28)))
+        return fail(c, AURORA_ERR_INVALID, "pass_paths must be 0 or in [2^16, 2^28]");
+    if (claim_paths && (claim_paths < 16 || claim_paths > 1024))
+        return fail(c, AURORA_ERR_INVALID, "claim_paths must be 0 or in [16, 1024]");
+    c->pass_paths = pass_paths;
+    c->claim_paths = claim_paths;
     return AURORA_OK;
 }
 

@@ -335,8 +335,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t
 // wave is not held by its longest path.  Same arithmetic per path as the
 // extend/shade pair, so results are unchanged.
 constexpr int kTailRefill = 4;
-// Paths a wave claims per pool atomic: about a sixteenth of a wave's share
-// of the pass, within [64, 128].  Large passes take 128 (Cornell, 33.5 M
+// Paths a wave claims per pool atomic (unless the context sets a size):
+// about a sixteenth of a wave's share of the pass, within [64, 128].  Large passes take 128 (Cornell, 33.5 M
 // paths: 1.5% faster than 64, fewer claims on the shared counters; 32 was 7%
 // slower); small passes of long paths keep 64, which balances the waves'
 // last claims (a 2 M-path stress render lost 5-15% at 128).
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 
     uint32_t* pool = b.counters + kPoolBase + kPoolStride * part;
     uint32_t b_next = 0, b_end = 0;  // wave-uniform, offsets within the range
     const uint32_t batch = __builtin_amdgcn_readfirstlane(
-        min(kTailBatchMax, max(kTailBatchMin, n / (gridDim.x * (kBlock / 64) * 16u))));
+        pa.claim ? pa.claim : min(kTailBatchMax, max(kTailBatchMin, n / (gridDim.x * (kBlock / 64) * 16u))));
     bool has = false, done = false;
     uint32_t path = 0, sid = 0, depth = 0, pixel = 0, sample = 0;  // pixel, sample: once per path
     v3 org = V(0, 0, 0), dir = V(0, 0, 0), beta = V(0, 0, 0);

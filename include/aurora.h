@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define AURORA_ABI_VERSION 4
+#define AURORA_ABI_VERSION 5
 
 /* ---- status codes ---------------------------------------------------- */
 #define AURORA_OK 0
@@ -233,6 +233,13 @@ void* aurora_stream(aurora_ctx* ctx);
  * only, never results. */
 int aurora_set_launch(aurora_ctx* ctx, int extend_blocks_per_cu,
                       int shade_blocks_per_cu, uint32_t tail_paths);
+/* Queue sizes: the most paths an automatic pass holds (spp_per_pass 0;
+ * default 0 = 2^27, else in [2^16, 2^28]; the path queues take 152 bytes per
+ * path), and the paths a wave of the fused tail claims per work-pool atomic
+ * (default 0 = by pass size, 64 to 128; else in [16, 1024]).  Changes speed
+ * only, never results: passes add their samples in order, and claims only
+ * decide which lane carries which path. */
+int aurora_set_queue(aurora_ctx* ctx, uint32_t pass_paths, uint32_t claim_paths);
 
 /* Tree walked by the render kernels: 0 = the binary LBVH (default; one
  * 64-byte child pair per step), 1 = the four-wide tree (one 128-byte node

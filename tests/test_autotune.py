@@ -74,3 +74,32 @@ def test_signature_separates_workloads():
     b = autotune.signature([(Sc(10), Cam(64), RS(8))])
     c = autotune.signature([(Sc(11), Cam(64), RS(4))])
     assert len({a, b, c}) == 3
+
+
+def test_refine_sweeps_queue_sizes_with_the_launch_shape_fixed(monkeypatch):
+    class Ctx(FakeCtx):
+        def __init__(self):
+            super().__init__()
+            self.queue = None
+
+        def set_queue(self, pass_paths=0, claim_paths=0):
+            self.queue = (pass_paths, claim_paths)
+
+    ctx = Ctx()
+    clock = [0.0]
+    seen = []
+
+    def run(graph):
+        seen.append((ctx.shape, ctx.tree, ctx.queue, graph))
+        p, c = ctx.queue
+        clock[0] += 1.0 + (0.0 if p == 1 << 25 else 0.3) + (0.0 if c == 256 else 0.1)
+
+    monkeypatch.setattr(autotune.time, "perf_counter", lambda: clock[0])
+    best = {"extend_blocks_per_cu": 16, "shade_blocks_per_cu": 8, "tail_paths": 1 << 20, "graph": True, "tree": 1,
+            "seconds": 5.0}
+    best2, table = autotune.refine(ctx, run, "sigQ", best, cache_path=None)
+    assert (best2["pass_paths"], best2["claim_paths"]) == (1 << 25, 256)
+    assert ctx.queue == (1 << 25, 256) and ctx.shape == (16, 8, 1 << 20) and ctx.tree == 1
+    assert len(table) == 9 and all(s[0] == (16, 8, 1 << 20) and s[1] == 1 and s[3] for s in seen)
+    # the library's choice (0, 0) is among the candidates
+    assert any(t["pass_paths"] == 0 and t["claim_paths"] == 0 for t in table)

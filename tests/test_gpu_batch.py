@@ -123,3 +123,33 @@ def test_pass_splits_do_not_change_images(ctx):
         render_batch(ctx, its, graph=graph)
         for k, (b, s) in enumerate(zip(bufs, singles)):
             assert np.array_equal(b.cpu().numpy(), s), (graph, k)
+
+
+def test_queue_sizes_do_not_change_images(ctx):
+    """aurora_set_queue: the most paths per pass (here 2^16, one sample of a
+    256^2 image per pass) and the paths per work-pool claim (16 and 1024)
+    change speed only: the sums equal the default render's bit for bit."""
+    from aurora_amd import scenes
+
+    print("queue sizes", flush=True)
+    g = ctx.upload(scenes.cornell_box())
+    g.build_bvh()
+    cam = scenes.cornell_camera(256)
+    rs = RenderSettings(spp=6, max_depth=50, seed=3, aov=True)
+    ctx.set_queue()
+    ref, ra, rn, st0 = g.render(cam, rs)
+    try:
+        for pass_paths, claim in [(1 << 16, 16), (1 << 16, 1024), (0, 16), (1 << 17, 0)]:
+            ctx.set_queue(pass_paths, claim)
+            rgb, alb, nrm, st = g.render(cam, rs)
+            assert np.array_equal(rgb, ref) and np.array_equal(alb, ra) and np.array_equal(nrm, rn)
+            assert st["segments"] == st0["segments"]
+            if pass_paths == 1 << 16:
+                assert st["passes"] == 6
+        with pytest.raises(ValueError, match="pass_paths"):
+            ctx.set_queue(100, 0)
+        with pytest.raises(ValueError, match="claim_paths"):
+            ctx.set_queue(0, 2000)
+    finally:
+        ctx.set_queue()
+        g.close()
