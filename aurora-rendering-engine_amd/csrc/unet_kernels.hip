@@ -29,12 +29,8 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16;
 
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ __forceinline__ u16 f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)((u >> 16) | 0x40);  // NaN stays NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                                      // round to nearest even
-    return (u16)(u >> 16);
-}
+// round to nearest even (gfx950's v_cvt_pk_bf16_f32; NaN stays NaN)
+__device__ __forceinline__ u16 f2bf(float f) { return __builtin_bit_cast(u16, (__bf16)f); }
 
 union Pack8 {
     uint4 u;
@@ -186,6 +182,33 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn in, const u16* __restric
 // distinct LDS banks, so an A fragment (16 pixels x 8 channels per k group)
 // reads without bank conflicts.
 __host__ __device__ constexpr int lds_pix_stride(int c) { return 2 * c + 16; }
+
+// Index i of a tile piece as (row r, pixel px, 16-byte group g), g fastest:
+// i = (r * radix_px + px) * radix_g + g.  Decomposed once for i = t, then
+// advanced by 256 (the block's threads) with two carries per step instead of
+// three integer divisions by runtime radices.
+struct Radix256 {
+    int g, px, r, dg, dpx, dr, rg, rpx;
+    __device__ Radix256(int t, int radix_g, int radix_px) : rg(radix_g), rpx(radix_px) {
+        const int q0 = t / radix_g;
+        g = t - q0 * radix_g;
+        px = q0 % radix_px;
+        r = q0 / radix_px;
+        const int qs = 256 / radix_g;  // 256 = qs * radix_g + dg, qs = dr * radix_px + dpx
+        dg = 256 - qs * radix_g;
+        dpx = qs % radix_px;
+        dr = qs / radix_px;
+    }
+    __device__ void step() {
+        g += dg;
+        const int c1 = g >= rg ? 1 : 0;
+        g -= c1 * rg;
+        px += dpx + c1;
+        const int c2 = px >= rpx ? 1 : 0;
+        px -= c2 * rpx;
+        r += dr + c2;
+    }
+};
 __host__ __device__ constexpr int lds_tile_bytes(int ca, int cb) {
     return 6 * 66 * lds_pix_stride(ca) + (cb ? 4 * 34 * lds_pix_stride(cb) : 0);
 }
@@ -209,33 +232,71 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
     const int ca = in.ca, cb = HB ? in.cb : 0, cin = ca + cb;
     const int psa = lds_pix_stride(ca), psb = lds_pix_stride(cb);
     unsigned char* const tb = tile + 6 * 66 * psa;  // the half-resolution part
-    // full-resolution part: rows y0-1 .. y0+4, pixels x0-1 .. x0+64
+    // full-resolution part: rows y0-1 .. y0+4, pixels x0-1 .. x0+64.  Each
+    // thread issues all its 16-byte loads before its first LDS store, so a
+    // block waits for memory once, not once per load (kStageLoads per
+    // thread cover 6 x 66 pixels of up to 64 channels: 3168 pieces)
     const int ga = ca >> 3;  // 16-byte channel groups per pixel
-    for (int i = threadIdx.x; i < 6 * 66 * ga; i += blockDim.x) {
-        const int g = i % ga, px = (i / ga) % 66, r = i / (ga * 66);
-        const int y = y0 - 1 + r, x = x0 - 1 + px;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (y >= 0 && y < H && x >= 0 && x < W)
-            v = *reinterpret_cast<const uint4*>(in.a + (((size_t)n * H + y) * W + x) * ca + 8 * g);
-        *reinterpret_cast<uint4*>(tile + (r * 66 + px) * psa + 16 * g) = v;
+    constexpr int kStageLoads = 13;
+    // Pieces outside the image load from a valid clamped address and are
+    // zeroed by a select, without branches; element offsets fit 32 bits (the
+    // host checks N*H*W*C < 2^31); the unrolled loops stop at the
+    // wave-uniform piece count.
+    {
+        const int total = 6 * 66 * ga;
+        Radix256 it((int)threadIdx.x, ga, 66);
+        uint4 v[kStageLoads];
+        int dst[kStageLoads];
+#pragma unroll
+        for (int j = 0; j < kStageLoads; ++j) {
+            v[j] = make_uint4(0, 0, 0, 0);
+            dst[j] = -1;
+            if (j * 256 < total) {  // wave-uniform
+                const int y = y0 - 1 + it.r, x = x0 - 1 + it.px;
+                const bool in_img = (unsigned)y < (unsigned)H && (unsigned)x < (unsigned)W;
+                const int off = ((n * H + (in_img ? y : 0)) * W + (in_img ? x : 0)) * ca + 8 * it.g;
+                const uint4 l = *reinterpret_cast<const uint4*>(in.a + off);
+                v[j] = in_img ? l : make_uint4(0, 0, 0, 0);
+                dst[j] = it.r < 6 ? (it.r * 66 + it.px) * psa + 16 * it.g : -1;
+                it.step();
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kStageLoads; ++j)
+            if (dst[j] >= 0) *reinterpret_cast<uint4*>(tile + dst[j]) = v[j];
     }
     // half-resolution part: the half-rows and half-pixels the full-resolution
     // rows y0-1 .. y0+4 and pixels x0-1 .. x0+64 fall in (arithmetic shifts:
-    // -1 maps to -1, outside)
+    // -1 maps to -1, outside); 4 x 34 pixels of up to 64 channels: 1088 pieces
     const int hy0 = (y0 - 1) >> 1, hx0 = (x0 - 1) >> 1, h2 = H >> 1, w2 = W >> 1;
     const int gb = cb >> 3;
-    for (int i = threadIdx.x; HB && i < 4 * 34 * gb; i += blockDim.x) {
-        const int g = i % gb, px = (i / gb) % 34, r = i / (gb * 34);
-        const int y = hy0 + r, x = hx0 + px;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (y >= 0 && y < h2 && x >= 0 && x < w2)
-            v = *reinterpret_cast<const uint4*>(in.b + (((size_t)n * h2 + y) * w2 + x) * cb + 8 * g);
-        *reinterpret_cast<uint4*>(tb + (r * 34 + px) * psb + 16 * g) = v;
+    if (HB) {
+        constexpr int kHalfLoads = 5;
+        const int total = 4 * 34 * gb;
+        Radix256 it((int)threadIdx.x, gb, 34);
+        uint4 v[kHalfLoads];
+        int dst[kHalfLoads];
+#pragma unroll
+        for (int j = 0; j < kHalfLoads; ++j) {
+            v[j] = make_uint4(0, 0, 0, 0);
+            dst[j] = -1;
+            if (j * 256 < total) {  // wave-uniform
+                const int y = hy0 + it.r, x = hx0 + it.px;
+                const bool in_img = (unsigned)y < (unsigned)h2 && (unsigned)x < (unsigned)w2;
+                const int off = ((n * h2 + (in_img ? y : 0)) * w2 + (in_img ? x : 0)) * cb + 8 * it.g;
+                const uint4 l = *reinterpret_cast<const uint4*>(in.b + off);
+                v[j] = in_img ? l : make_uint4(0, 0, 0, 0);
+                dst[j] = it.r < 4 ? (it.r * 34 + it.px) * psb + 16 * it.g : -1;
+                it.step();
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kHalfLoads; ++j)
+            if (dst[j] >= 0) *reinterpret_cast<uint4*>(tb + dst[j]) = v[j];
     }
     __syncthreads();
     const int y = y0 + wave;
     if (y >= H) return;  // no barrier follows
-    const int K = 9 * cin;
     const int m = lane & 15;   // pixel of the A fragment within its row tile
     const int kg = lane >> 4;  // k group of 8
     floatx4 acc[4][NT];
@@ -243,15 +304,29 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[mt][t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    bf16x8 a[4], b[NT];
-    auto load = [&](int k0, bf16x8* fa, bf16x8* fb) {
-        const int k = k0 + 8 * kg;
-        if (k < K) {
-            const int tap = k / cin, c = k - tap * cin;
-            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    // This lane's k group (k = k0 + 8 kg) as tap row ty, tap column tx and
+    // channel c, advanced by 32 per chunk with a few integer adds (a chunk
+    // crosses at most two taps: cin >= 16); ty == 3 is past the last tap.
+    int ty = 0, tx = 0, c = 8 * kg;
+    auto advance = [&](int step) {
+        c += step;
+        while (c >= cin) {
+            c -= cin;
+            if (++tx == 3) {
+                tx = 0;
+                ++ty;
+            }
+        }
+    };
+    advance(0);
+    const u16* wrow[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) wrow[t] = wk + (size_t)(t * 16 + m) * kpad + 8 * kg;
+    auto load = [&](bf16x8* fa, bf16x8* fb) {
+        if (ty < 3) {
             if (!HB || c < ca) {
-                // tile row of input row y + dy, tile pixel of x0 + m + dx
-                const unsigned char* src = tile + ((wave + dy + 1) * 66 + m + dx + 1) * psa + 2 * c;
+                // tile row of input row y + ty - 1, tile pixel of x0 + m + tx - 1
+                const unsigned char* src = tile + ((wave + ty) * 66 + m + tx) * psa + 2 * c;
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt) {
                     Pack8 v;
@@ -259,11 +334,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
                     fa[mt] = v.v;
                 }
             } else {
-                // input pixel (y + dy, x) read at half resolution
-                const int hr = ((y + dy) >> 1) - hy0;
+                // input pixel (y + ty - 1, x) read at half resolution
+                const int hr = ((y + ty - 1) >> 1) - hy0;
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt) {
-                    const int hp = ((x0 + 16 * mt + m + dx) >> 1) - hx0;
+                    const int hp = ((x0 + 16 * mt + m + tx - 1) >> 1) - hx0;
                     Pack8 v;
                     v.u = *reinterpret_cast<const uint4*>(tb + (hr * 34 + hp) * psb + 2 * (c - ca));
                     fa[mt] = v.v;
@@ -276,25 +351,31 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             Pack8 w;
-            w.u = *reinterpret_cast<const uint4*>(wk + (size_t)(t * 16 + m) * kpad + k);
+            w.u = *reinterpret_cast<const uint4*>(wrow[t]);
             fb[t] = w.v;
+            wrow[t] += 32;
         }
+        advance(32);
     };
-    load(0, a, b);
-    for (int k0 = 0; k0 < kpad; k0 += 32) {
-        bf16x8 na[4], nb[NT];
-        const bool more = k0 + 32 < kpad;
-        if (more) load(k0 + 32, na, nb);
+    auto mma = [&](const bf16x8* fa, const bf16x8* fb) {
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int t = 0; t < NT; ++t) acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[t], a[mt], acc[mt][t], 0, 0, 0);
-        if (more) {
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt) a[mt] = na[mt];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) b[t] = nb[t];
-        }
+            for (int t = 0; t < NT; ++t)
+                acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[t], fa[mt], acc[mt][t], 0, 0, 0);
+    };
+    // two fragment sets in turn: chunk q + 1 loads while chunk q multiplies
+    bf16x8 a0[4], b0[NT], a1[4], b1[NT];
+    load(a0, b0);
+    for (int k0 = 0;; k0 += 64) {
+        const bool more1 = k0 + 32 < kpad;
+        if (more1) load(a1, b1);
+        mma(a0, b0);
+        if (!more1) break;
+        const bool more2 = k0 + 64 < kpad;
+        if (more2) load(a0, b0);
+        mma(a1, b1);
+        if (!more2) break;
     }
     // the MFMAs computed each tile transposed (weights as the A operand),
     // so this lane holds channels co .. co+3 of pixel x: one 8-byte (BF16)
@@ -673,8 +754,11 @@ Status conv3x3_forward(const ConvIn& in, const uint16_t* wk_all, int kpad, const
         return Status{AURORA_ERR_INVALID, "conv3x3: channel counts must be multiples of 16"};
     // the LDS-tiled kernel where its tiles fit 80 KB (of the 160 KB of LDS per
     // CU; every layer of the network: the largest, the level-2 decoder input
-    // of 64 + 64 channels, takes 76.6 KB), else the gather form
-    const int tr = lds_tile_bytes(in.ca, in.cb) <= 81920 ? 4 : 0;
+    // of 64 + 64 channels, takes 76.6 KB) and each part has at most 64
+    // channels, else the gather form
+    // (its staging loads cover up to 64 channels per part in 256 threads)
+    const bool small = (size_t)N * H * W * std::max(in.ca, in.cb) < (1ull << 31);  // 32-bit staging offsets
+    const int tr = lds_tile_bytes(in.ca, in.cb) <= 81920 && in.ca <= 64 && in.cb <= 64 && small ? 4 : 0;
     const unsigned grid = (unsigned)(((size_t)N * H * ((W + 63) / 64) + 3) / 4);  // 4 row segments per block
     const unsigned grid_lds = tr ? (unsigned)((size_t)N * ((H + 3) / 4) * ((W + 63) / 64)) : 0u;
     const size_t lds = tr ? (size_t)lds_tile_bytes(in.ca, in.cb) : 0;

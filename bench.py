@@ -131,7 +131,8 @@ def stress_section(ctx, n_rays=1 << 22, render_spp=64, render_only=False):
     ralg = 32 * rc["nodes_fetched"] + 16 * rc["spheres_fetched"] + 48 * rc["quads_fetched"]
     render = {"alg_traversal_bytes_binary": ralg}
     for mode, tp, wide in (("wavefront_tail_1M", 1 << 20, 0), ("persistent_tail", TAIL_ALWAYS, 0),
-                           ("persistent_tail_four_wide", TAIL_ALWAYS, 1)):
+                           ("persistent_tail_four_wide", TAIL_ALWAYS, 1), ("wavefront_tail_1M_four_wide", 1 << 20, 1),
+                           ("wavefront_four_wide", 0, 1)):
         ctx.set_launch(tail_paths=tp)
         ctx.set_render_tree(wide)
         scene.render_device(cam, rs, rgb)
