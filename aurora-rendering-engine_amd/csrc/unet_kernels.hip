@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvIn in, const u16* __restric
 // reads without bank conflicts.
 __host__ __device__ constexpr int lds_pix_stride(int c) { return 2 * c + 16; }
 
-// Index i of a tile piece as (row r, pixel px, 16-byte group g), g fastest:
+// Index i in a mixed radix of three digits, the first fastest:
 // i = (r * radix_px + px) * radix_g + g.  Decomposed once for i = t, then
 // advanced by 256 (the block's threads) with two carries per step instead of
 // three integer divisions by runtime radices.
@@ -244,7 +244,10 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
     // wave-uniform piece count.
     {
         const int total = 6 * 66 * ga;
-        Radix256 it((int)threadIdx.x, ga, 66);
+        // pixel fastest: consecutive lanes write one channel group of
+        // consecutive pixels, whose addresses (stride 2*C + 16 bytes) fall in
+        // distinct LDS banks
+        Radix256 it((int)threadIdx.x, 66, 6);  // (pixel, row, group)
         uint4 v[kStageLoads];
         int dst[kStageLoads];
 #pragma unroll
@@ -252,12 +255,13 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
             v[j] = make_uint4(0, 0, 0, 0);
             dst[j] = -1;
             if (j * 256 < total) {  // wave-uniform
-                const int y = y0 - 1 + it.r, x = x0 - 1 + it.px;
+                const int px = it.g, r = it.px, g = it.r;
+                const int y = y0 - 1 + r, x = x0 - 1 + px;
                 const bool in_img = (unsigned)y < (unsigned)H && (unsigned)x < (unsigned)W;
-                const int off = ((n * H + (in_img ? y : 0)) * W + (in_img ? x : 0)) * ca + 8 * it.g;
+                const int off = ((n * H + (in_img ? y : 0)) * W + (in_img ? x : 0)) * ca + 8 * (g < ga ? g : 0);
                 const uint4 l = *reinterpret_cast<const uint4*>(in.a + off);
                 v[j] = in_img ? l : make_uint4(0, 0, 0, 0);
-                dst[j] = it.r < 6 ? (it.r * 66 + it.px) * psa + 16 * it.g : -1;
+                dst[j] = g < ga ? (r * 66 + px) * psa + 16 * g : -1;
                 it.step();
             }
         }
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
     if (HB) {
         constexpr int kHalfLoads = 5;
         const int total = 4 * 34 * gb;
-        Radix256 it((int)threadIdx.x, gb, 34);
+        Radix256 it((int)threadIdx.x, 34, 4);  // (pixel, row, group)
         uint4 v[kHalfLoads];
         int dst[kHalfLoads];
 #pragma unroll
@@ -281,12 +285,13 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
             v[j] = make_uint4(0, 0, 0, 0);
             dst[j] = -1;
             if (j * 256 < total) {  // wave-uniform
-                const int y = hy0 + it.r, x = hx0 + it.px;
+                const int px = it.g, r = it.px, g = it.r;
+                const int y = hy0 + r, x = hx0 + px;
                 const bool in_img = (unsigned)y < (unsigned)h2 && (unsigned)x < (unsigned)w2;
-                const int off = ((n * h2 + (in_img ? y : 0)) * w2 + (in_img ? x : 0)) * cb + 8 * it.g;
+                const int off = ((n * h2 + (in_img ? y : 0)) * w2 + (in_img ? x : 0)) * cb + 8 * (g < gb ? g : 0);
                 const uint4 l = *reinterpret_cast<const uint4*>(in.b + off);
                 v[j] = in_img ? l : make_uint4(0, 0, 0, 0);
-                dst[j] = it.r < 4 ? (it.r * 34 + it.px) * psb + 16 * it.g : -1;
+                dst[j] = g < gb ? (r * 34 + px) * psb + 16 * g : -1;
                 it.step();
             }
         }
