@@ -563,20 +563,32 @@ __global__ void k_mask_bf16(const float* __restrict__ grad, const u16* __restric
     out[i] = f2bf(v);
 }
 
+// 2x2 average of BF16 activations, eight channels (one 16-byte load of
+// each of the four pixels) per thread; C is a multiple of 8
 __global__ void k_avgpool(const u16* __restrict__ in, int N, int H, int W, int C, u16* __restrict__ out) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const int h2 = H / 2, w2 = W / 2;
-    const size_t total = (size_t)N * h2 * w2 * C;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;  // group of 8 output channels
+    const int h2 = H / 2, w2 = W / 2, cg = C / 8;
+    const size_t total = (size_t)N * h2 * w2 * cg;
     if (i >= total) return;
-    const int c = i % C;
-    size_t q = i / C;
-    const int x = q % w2;
+    const int g = (int)(i % cg);
+    size_t q = i / cg;
+    const int x = (int)(q % w2);
     q /= w2;
-    const int y = q % h2;
-    const int n = q / h2;
-    auto at = [&](int yy, int xx) { return bf2f(in[(((size_t)n * H + yy) * W + xx) * C + c]); };
-    const float s = (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + (at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
-    out[i] = f2bf(0.25f * s);
+    const int y = (int)(q % h2);
+    const int n = (int)(q / h2);
+    auto at = [&](int yy, int xx) {
+        Pack8 p;
+        p.u = *reinterpret_cast<const uint4*>(in + (((size_t)n * H + yy) * W + xx) * C + 8 * g);
+        return p;
+    };
+    const Pack8 a = at(2 * y, 2 * x), b = at(2 * y, 2 * x + 1), c = at(2 * y + 1, 2 * x), d = at(2 * y + 1, 2 * x + 1);
+    Pack8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float s = (bf2f(a.h[k]) + bf2f(b.h[k])) + (bf2f(c.h[k]) + bf2f(d.h[k]));
+        o.h[k] = f2bf(0.25f * s);
+    }
+    *reinterpret_cast<uint4*>(out + (((size_t)n * h2 + y) * w2 + x) * C + 8 * g) = o.u;
 }
 
 // gin[p] += 0.25 * gout[parent(p)]
@@ -843,7 +855,8 @@ Status mask_to_bf16(const float* grad, const uint16_t* act, size_t n, uint16_t* 
     return Status::ok();
 }
 Status avgpool2(const uint16_t* in, int N, int H, int W, int C, uint16_t* out, hipStream_t s) {
-    k_avgpool<<<nblocks((size_t)N * (H / 2) * (W / 2) * C), 256, 0, s>>>(in, N, H, W, C, out);
+    if (C % 8) return Status{AURORA_ERR_INVALID, "avgpool: channel count must be a multiple of 8"};
+    k_avgpool<<<nblocks((size_t)N * (H / 2) * (W / 2) * (C / 8)), 256, 0, s>>>(in, N, H, W, C, out);
     AUR_HIP(hipGetLastError());
     return Status::ok();
 }
