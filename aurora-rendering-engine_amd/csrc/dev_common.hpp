@@ -198,6 +198,24 @@ __device__ __forceinline__ float4 ldg4(const float4* p, size_t i) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ uint32_t ldg1(const uint32_t* p, size_t i) { return ((aur_g1p)p)[i]; }
+// Loads at a wave-uniform address from an array that no kernel writes while
+// the reading kernel runs.  Through the constant address space the compiler
+// issues them as scalar loads: no vector memory instruction, no wait on the
+// vector memory counter, and the values land in scalar registers that the
+// vector instructions read directly.  The pointer and index must be
+// wave-uniform (uniform_ptr, readfirstlane).
+typedef __attribute__((address_space(4))) const aur_f4v* aur_c4p;
+__device__ __forceinline__ float4 ldu4(const float4* p, size_t i) {
+    const aur_f4v v = ((aur_c4p)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+template <class T>
+__device__ __forceinline__ const T* uniform_ptr(const T* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (const T*)(((uint64_t)hi << 32) | lo);
+}
 
 struct DevScene {
     const float4* spheres;        // (centre, radius)

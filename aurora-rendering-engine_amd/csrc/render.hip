@@ -26,6 +26,26 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// The scene record of a path.  A single-scene launch reads recs[0] through
+// the constant address space: nothing writes the table while a render
+// kernel runs, so its fields arrive by scalar loads and the scene's pointers
+// and counts stay in scalar registers.
+template <bool ONE>
+__device__ __forceinline__ SceneRec scene_rec(const SceneRec* recs, uint32_t sid) {
+    if constexpr (ONE) {
+        static_assert(sizeof(SceneRec) % 4 == 0, "dword copy");
+        typedef const __attribute__((address_space(4))) uint32_t* cu32p;
+        SceneRec out;
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+        const cu32p src = (cu32p)recs;
+#pragma unroll
+        for (size_t i = 0; i < sizeof(SceneRec) / 4; ++i) dst[i] = src[i];
+        return out;
+    } else {
+        return recs[sid];
+    }
+}
+
 // last record whose offset (path_off or pix_off) is <= x
 template <bool PIX>
 __device__ __forceinline__ uint32_t find_rec(const SceneRec* recs, uint32_t n, uint32_t x) {
@@ -81,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Pass pa, float4* qo, float4* 
     if (p < kPools) counters[kPoolBase + kPoolStride * p] = 0u;  // extend work pools
     if (p >= n) return;
     const uint32_t sid = ONE ? 0u : find_rec<false>(pa.recs, pa.n_recs, p);
-    const SceneRec& r = pa.recs[sid];
+    const SceneRec r = scene_rec<ONE>(pa.recs, sid);
     const uint32_t local = p - r.path_off;
     const uint32_t s_local = local / r.WH;
     const uint32_t pixel = local - s_local * r.WH;
@@ -296,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t
             const float2 H = b.hit[i];
             path = __float_as_uint(O.w);
             sid = ONE ? 0u : __float_as_uint(D.w);
-            const SceneRec& r = pa.recs[sid];
+            const SceneRec r = scene_rec<ONE>(pa.recs, sid);
             uint32_t pixel, sample;
             path_pixel(r, path, pixel, sample);
             o = shade_core(r, b, xyz(O), xyz(D), xyz(B), H.x, __float_as_int(H.y), path, pixel, sample, depth);
@@ -413,7 +433,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 
                     dir = xyz(D);
                     beta = xyz(B);
                     depth = depth0;
-                    path_pixel(pa.recs[sid], path, pixel, sample);
+                    path_pixel(scene_rec<ONE>(pa.recs, sid), path, pixel, sample);
                     has = true;
                 }
             }
@@ -422,7 +442,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 
         if (__ballot(has || !done) == 0) break;
         if (!has) continue;
         // one segment of this lane's path
-        const SceneRec& r = pa.recs[sid];
+        const SceneRec r = scene_rec<ONE>(pa.recs, sid);
         ++segs;
         float t;
         int prim;

@@ -180,16 +180,18 @@ __device__ __forceinline__ void traverse_closest_wide(const DevScene& sc, v3 o, 
 // tree walks' one.
 __device__ __forceinline__ void traverse_closest_flat(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
                                                       float& t_out, int& prim_out) {
-    const float4* const F = sc.flat;
-    const float4* const S = sc.spheres;
-    const float4* const Qd = sc.quads;
-    const uint32_t n_spheres = sc.n_spheres, n_flat = sc.n_flat;
-    const uint32_t n_prims = sc.n_spheres + sc.n_quads;
+    // one scene per launch: the scene's fields are wave-uniform
+    const float4* const F = uniform_ptr(sc.flat);
+    const float4* const S = uniform_ptr(sc.spheres);
+    const float4* const Qd = uniform_ptr(sc.quads);
+    const uint32_t n_spheres = __builtin_amdgcn_readfirstlane(sc.n_spheres);
+    const uint32_t n_flat = __builtin_amdgcn_readfirstlane(sc.n_flat);
+    const uint32_t n_prims = n_spheres + __builtin_amdgcn_readfirstlane(sc.n_quads);
     const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     float best_t = tmax;
     uint32_t best_id = 0x7fffffffu;
     for (uint32_t k = 0; k < n_flat; ++k) {
-        const float4 lo = ldg4(F, 2 * (size_t)k), hi = ldg4(F, 2 * (size_t)k + 1);
+        const float4 lo = ldu4(F, 2 * (size_t)k), hi = ldu4(F, 2 * (size_t)k + 1);
         float tn = 0.0f;
         const bool hb = slab(lo, hi, o, inv, tmin, best_t, tn);
         const uint32_t id = __float_as_uint(lo.w) & ~AURORA_NODE_LEAF;
@@ -199,10 +201,10 @@ __device__ __forceinline__ void traverse_closest_flat(const DevScene& sc, v3 o, 
         float t = 0.0f;
         bool h;
         if (q < n_spheres) {
-            h = hit_sphere_t(ldg4(S, q), o, d, tmin, tmax, t);
+            h = hit_sphere_t(ldu4(S, q), o, d, tmin, tmax, t);
         } else {
             const size_t b = 3 * (size_t)(q - n_spheres);
-            h = hit_quad_t(ldg4(Qd, b), ldg4(Qd, b + 1), ldg4(Qd, b + 2), o, d, tmin, tmax, t);
+            h = hit_quad_t(ldu4(Qd, b), ldu4(Qd, b + 1), ldu4(Qd, b + 2), o, d, tmin, tmax, t);
         }
         const bool take = hb & valid & h & (t >= tn) & ((t < best_t) | ((t == best_t) & (id < best_id)));
         best_t = take ? t : best_t;
