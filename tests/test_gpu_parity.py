@@ -303,6 +303,41 @@ def test_render_radiance_bit_exact(ctx, name, mk, mkcam, rs):
         assert eq.all()
 
 
+# Flat-walk scene sizes: one sphere, one triangle, spheres and triangles
+# mixed up to the flat array's limit of 256 primitives, and one past it
+# (257: no flat array, the four-wide tree stands in).  Primitives five times
+# the stress scene's size (a lone triangle forty times), so that a 40x24
+# image hits them; checker
+# and Perlin textures.
+FLAT_SIZES = [(1, 0), (0, 1), (200, 56), (0, 256), (256, 0), (129, 128)]
+
+
+@pytest.mark.parametrize("ns,nt", FLAT_SIZES, ids=[f"s{a}_t{b}" for a, b in FLAT_SIZES])
+def test_flat_walk_sizes_bit_exact(ctx, ns, nt):
+    say("flat walk", ns, "spheres", nt, "triangles")
+    arr = scenes.stress_scene(ns, nt, seed=11)
+    arr.spheres[:, 3] *= 5.0
+    arr.quads[:, 3:] *= 40.0 if nt == 1 else 5.0  # a lone triangle large enough to be seen
+    cam = scenes.stress_camera(40, 24)
+    rs = RenderSettings(spp=4, max_depth=16, background=(0.7, 0.8, 1.0), tmin=1e-4)
+    g, gs, o, os_ = both(ctx, arr)
+    orgb, _, _, segs = o.render(cam, rs)
+    hits = (orgb != orgb[0, 0]).any(-1).mean()
+    for tail, tree in ((1 << 30, 2), (2048, 2), (1 << 30, 1)):
+        ctx.set_launch(tail_paths=tail)
+        ctx.set_render_tree(tree)
+        try:
+            grgb, _, _, st = g.render(cam, rs)
+        finally:
+            ctx.set_launch()
+            ctx.set_render_tree(0)
+        say("flat walk", ns, nt, "tail", tail, "tree", tree, "segments gpu/cpu", st["segments"], segs,
+            "pixels off the background", hits)
+        assert st["segments"] == segs
+        assert np.array_equal(grgb.view(np.uint32), orgb.view(np.uint32))
+    assert segs > cam.image_width * cam.image_height * rs.spp  # some paths bounce
+
+
 def test_render_textures_bit_exact(ctx):
     """Image textures on a sphere (spherical u, v from the series atan2 and
     acos), a parallelogram, a triangle and an emitter, with checker, Perlin,
