@@ -631,6 +631,10 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
         pa.claim = c->claim_paths;
         pa.n_paths = path_off;
         pa.n_pixels = pix_off;
+        // the fused tail carries the pass from the first bounce (the pass
+        // fits the tail switch): the raygen queue needs no throughput, and
+        // the tail writes every path's radiance
+        pa.lean_queue = path_off <= tail_paths ? 1u : 0u;
         same_shape = sig == prev_sig;
         prev_sig = sig;
         return Status::ok();

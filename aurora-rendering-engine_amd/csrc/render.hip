@@ -110,6 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Pass pa, float4* qo, float4* 
     camera_ray(r.cam, pixel, i, j, r.sample_base + s_local, r.seed, org, dir);
     qo[p] = make_float4(org.x, org.y, org.z, __uint_as_float(p));
     qd[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(sid));
+    if (pa.lean_queue) return;  // the fused tail sets throughput 1 and writes every L
     qb[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -185,6 +186,7 @@ struct ShadeBufs {
 struct ShadeOut {
     v3 p, nd, beta;
     bool alive;
+    bool emitted;  // L[path] written (background or emitter)
 };
 
 // The pixel and sample of path id `path` (sample-major within its scene).
@@ -205,10 +207,12 @@ __device__ __forceinline__ ShadeOut shade_core(const SceneRec& r, const ShadeBuf
     const bool aov = r.aov && depth == 0;
     ShadeOut out;
     out.alive = false;
+    out.emitted = false;
     const v3 bg = V(r.bg[0], r.bg[1], r.bg[2]);
     if (prim < 0) {
         const v3 e = beta * bg;
         b.L[path] = make_float4(e.x, e.y, e.z, 0.0f);
+        out.emitted = true;
         if (aov) {
             const v3 a = clamp01(bg);
             b.aov_a[path] = make_float4(a.x, a.y, a.z, 0.0f);
@@ -253,6 +257,7 @@ __device__ __forceinline__ ShadeOut shade_core(const SceneRec& r, const ShadeBuf
         const v3 e0 = texture_value(sc, m.texture, tu, tv, p);
         const v3 e = beta * e0;
         b.L[path] = make_float4(e.x, e.y, e.z, 0.0f);
+        out.emitted = true;
         att = clamp01(e0);
         scatter = false;
     } else if (m.type == AURORA_MAT_LAMBERTIAN) {
@@ -426,12 +431,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 
                     done = true;
                 } else if (rank < avail) {
                     const uint32_t i = r_lo + b_next + rank;
-                    const float4 O = b.qo[i], D = b.qd[i], B = b.qb[i];
+                    const float4 O = b.qo[i], D = b.qd[i];
                     path = __float_as_uint(O.w);
                     sid = ONE ? 0u : __float_as_uint(D.w);
                     org = xyz(O);
                     dir = xyz(D);
-                    beta = xyz(B);
+                    beta = pa.lean_queue ? V(1.0f, 1.0f, 1.0f) : xyz(b.qb[i]);
                     depth = depth0;
                     path_pixel(scene_rec<ONE>(pa.recs, sid), path, pixel, sample);
                     has = true;
@@ -456,6 +461,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 
                                     spill_stride, t, prim);
         const ShadeOut o = shade_core(r, b, org, dir, beta, t, prim, path, pixel, sample, depth);
         if (!o.alive) {
+            // a path absorbed or stopped by the depth limit adds nothing (the
+            // raygen kernel zeroes L unless the queue is lean)
+            if (pa.lean_queue && !o.emitted) b.L[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             has = false;
         } else {
             org = o.p;
