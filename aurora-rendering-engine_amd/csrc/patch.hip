@@ -95,39 +95,6 @@ HTri host_tri(const aurora_patch_triangle& t, const aurora_patch_material* mats)
     return r;
 }
 
-// barycentric (u, v, w) of q in t's plane; (-1, -1, -1) for a degenerate t
-void barycentric(P3 q, const HTri& t, double& u, double& v, double& w) {
-    const P3 a = t.p[1] - t.p[0], b = t.p[2] - t.p[0], c = q - t.p[0];
-    const double aa = dot3(a, a), ab = dot3(a, b), bb = dot3(b, b), ca = dot3(c, a), cb = dot3(c, b);
-    const double den = aa * bb - ab * ab;
-    if (std::fabs(den) < 1e-9) {
-        u = v = w = -1.0;
-        return;
-    }
-    v = (bb * ca - ab * cb) / den;
-    w = (aa * cb - ab * ca) / den;
-    u = 1.0 - v - w;
-}
-
-// the eye -> point segment meets vp's plane strictly inside (t in (1e-7,
-// 1 - 1e-7)); the meeting point's vp uv (extrapolated outside vp) and
-// whether it lies inside vp (barycentric coordinates >= -1e-8: the
-// reference's pointInTriangle3D of the same point, rt10.cpp:195-199)
-bool project(P3 eye, P3 point, const HTri& vp, P3 n, P2& uv, bool& inside) {
-    const P3 d = point - eye;
-    const double den = dot3(n, d);
-    if (std::fabs(den) < 1e-9) return false;
-    const double t = dot3(n, vp.p[0] - eye) / den;
-    if (!(t > 1e-7 && t < 1.0 - 1e-7)) return false;
-    const P3 hit = eye + d * t;
-    double a, b, c;
-    barycentric(hit, vp, a, b, c);
-    if (a < -1e6 || b < -1e6 || c < -1e6) return false;
-    uv = vp.uv[0] * a + vp.uv[1] * b + vp.uv[2] * c;
-    inside = a >= -1e-8 && b >= -1e-8 && c >= -1e-8;
-    return true;
-}
-
 struct UVPair {
     P2 dst, src;
 };
@@ -188,7 +155,15 @@ struct PatchTile {
 struct Plan {
     const std::vector<HTri>* scene = nullptr;
     const std::vector<P3>* verts = nullptr;  // distinct vertex positions of the scene
-    std::vector<uint8_t> vin;                // per distinct vertex: projects inside the current viewport
+    // per distinct vertex, seen from an eye through a viewport triangle:
+    // project()'s result (ok: the segment meets the plane; uv; in: inside)
+    struct VProj {
+        P2 uv;
+        bool ok, in;
+    };
+    // one table per recursion level: a level's table stays valid while the
+    // textures below it are planned
+    std::vector<std::vector<VProj>> proj;
     aurora_patch_config cfg{};
     std::vector<PatchTex> texs;
     std::vector<int> tex_depth;
@@ -205,48 +180,53 @@ struct Plan {
         double dist;  // centroid distance from the eye
         const HTri* t;
     };
-    // vin[v] = distinct vertex v projects inside vp seen from eye: project()
-    // and the inside test, with the terms that depend on vp alone (normal,
-    // plane offset, barycentric Gram terms) computed once, and the per-vertex
-    // tests in branch-free form (the same operations on the same values as
-    // project(), so the same decisions).
-    void project_all(P3 eye, const HTri& vp) {
+    // out[v]: distinct vertex v seen from eye through vp.  ok: the eye ->
+    // vertex segment meets vp's plane strictly inside (t in (1e-7,
+    // 1 - 1e-7)) and the meeting point's barycentric coordinates in vp are
+    // above -1e6 ((-1, -1, -1) for a degenerate vp); uv: that point's vp uv
+    // (extrapolated outside vp); in: it lies inside vp (coordinates >= -1e-8:
+    // the reference's pointInTriangle3D of the same point, rt10.cpp:195-199).
+    // The terms that depend on vp alone (normal, plane offset, barycentric
+    // Gram terms) are computed once; the per-vertex operations are the
+    // reference's, in its order.
+    void project_all(P3 eye, const HTri& vp, std::vector<VProj>& out) {
         const size_t nv = verts->size();
-        vin.resize(nv);
+        out.resize(nv);
         const P3 n = vp.normal();
         const double num = dot3(n, vp.p[0] - eye);
         const P3 a = vp.p[1] - vp.p[0], b = vp.p[2] - vp.p[0];
         const double aa = dot3(a, a), ab = dot3(a, b), bb = dot3(b, b);
         const double gden = aa * bb - ab * ab;
-        if (std::fabs(gden) < 1e-9) {
-            // degenerate viewport: barycentric (-1, -1, -1), never inside
-            std::fill(vin.begin(), vin.end(), uint8_t(0));
-            return;
-        }
+        const bool degenerate = std::fabs(gden) < 1e-9;  // barycentric (-1, -1, -1)
         const P3* V = verts->data();
-        uint8_t* out = vin.data();
         for (size_t v = 0; v < nv; ++v) {
             const P3 d = V[v] - eye;
             const double den = dot3(n, d);
             const double t = num / den;
             const P3 c = (eye + d * t) - vp.p[0];
             const double ca = dot3(c, a), cb = dot3(c, b);
-            const double bv = (bb * ca - ab * cb) / gden;
-            const double bw = (aa * cb - ab * ca) / gden;
-            const double bu = 1.0 - bv - bw;
-            out[v] = uint8_t((std::fabs(den) >= 1e-9) & (t > 1e-7) & (t < 1.0 - 1e-7) & (bu >= -1e-8) &
-                             (bv >= -1e-8) & (bw >= -1e-8));
+            const double bv = degenerate ? -1.0 : (bb * ca - ab * cb) / gden;
+            const double bw = degenerate ? -1.0 : (aa * cb - ab * ca) / gden;
+            const double bu = degenerate ? -1.0 : 1.0 - bv - bw;
+            VProj& o = out[v];
+            o.ok = (std::fabs(den) >= 1e-9) & (t > 1e-7) & (t < 1.0 - 1e-7) & !(bu < -1e6) & !(bv < -1e6) &
+                   !(bw < -1e6);
+            o.uv = vp.uv[0] * bu + vp.uv[1] * bv + vp.uv[2] * bw;
+            o.in = o.ok & (bu >= -1e-8) & (bv >= -1e-8) & (bw >= -1e-8);
         }
     }
 
     // A triangle is visible when one of its vertices projects inside vp; a
     // vertex position shared by several triangles is tested once (the same
-    // operations on the same values: the same decision).
-    std::vector<Seen> visible(P3 eye, const HTri& vp) {
-        project_all(eye, vp);
+    // operations on the same values: the same decision).  The projections
+    // stay in proj[level] for clip().
+    std::vector<Seen> visible(P3 eye, const HTri& vp, int level) {
+        if (proj.size() <= size_t(level)) proj.resize(size_t(std::max(level, cfg.max_depth)) + 2);
+        std::vector<VProj>& pv = proj[size_t(level)];
+        project_all(eye, vp, pv);
         std::vector<Seen> out;
         for (const HTri& t : *scene)
-            if (vin[t.vi[0]] || vin[t.vi[1]] || vin[t.vi[2]]) {
+            if (pv[t.vi[0]].in || pv[t.vi[1]].in || pv[t.vi[2]].in) {
                 const P3 dc = t.cen - eye;
                 out.push_back({std::sqrt(dot3(dc, dc)), &t});
             }
@@ -257,14 +237,16 @@ struct Plan {
         return out;
     }
 
-    // clipped projection of t into vp's uv triangle (in ca); false when a
-    // vertex does not project or fewer than three vertices remain
-    bool clip(P3 eye, const HTri& t, const HTri& vp) {
-        const P3 n = vp.normal();
+    // clipped projection of t into vp's uv triangle (in ca), from the
+    // projections visible(eye, vp, level) made; false when a vertex does not
+    // project or fewer than three vertices remain
+    bool clip(const HTri& t, const HTri& vp, int level) {
+        const std::vector<VProj>& pv = proj[size_t(level)];
         ca.resize(3);
         for (int i = 0; i < 3; ++i) {
-            bool inside;
-            if (!project(eye, t.p[i], vp, n, ca[i].dst, inside)) return false;
+            const VProj& v = pv[t.vi[i]];
+            if (!v.ok) return false;
+            ca[i].dst = v.uv;
             ca[i].src = t.uv[i];
         }
         P2 c0 = vp.uv[0], c1 = vp.uv[1], c2 = vp.uv[2];
@@ -339,11 +321,11 @@ struct Plan {
         const P3 n = cur.normal();
         const P3 eye2 = eye - n * (2.0 * dot3(n, eye - cur.p[0]));  // the eye mirrored across cur
         std::vector<PatchWarp> mine;
-        for (const Seen& seen : visible(eye2, cur)) {
+        for (const Seen& seen : visible(eye2, cur, depth + 1)) {
             const HTri* t = seen.t;
             if (status.bad()) break;
             if (t->id == cur.id) continue;
-            if (!clip(eye2, *t, cur)) continue;
+            if (!clip(*t, cur, depth + 1)) continue;
             const double px = area_uv(ca) * double(tw) * double(th);
             if (px < 0.5) continue;
             const int res = clampi(int(std::lround(std::sqrt(px) * 1.2)), cfg.min_tex_res, cfg.max_tex_res);
@@ -377,6 +359,16 @@ struct FramePlan {
     uint32_t begin[2] = {0, 0}, end[2] = {0, 0};
     uint64_t pool_texels = 0, warp_texels = 0;
     int max_level = -1;
+    // empty tables that keep their storage: a frame reuses the last one's
+    // pages instead of faulting in fresh ones
+    void reset() {
+        texs.clear();
+        tex_depth.clear();
+        warps.clear();
+        begin[0] = begin[1] = end[0] = end[1] = 0;
+        pool_texels = warp_texels = 0;
+        max_level = -1;
+    }
 };
 
 // max_threads 0: the library's choice
@@ -419,8 +411,8 @@ Status plan_frame(std::vector<HTri>& scene, const aurora_patch_config& cfg, P3 e
     top.cfg = cfg;
     std::vector<Top> items;
     for (int k = 0; k < 2; ++k)
-        for (const Plan::Seen& seen : top.visible(eye, vp[k])) {
-            if (!top.clip(eye, *seen.t, vp[k])) continue;
+        for (const Plan::Seen& seen : top.visible(eye, vp[k], 0)) {
+            if (!top.clip(*seen.t, vp[k], 0)) continue;
             const double px = Plan::area_uv(top.ca) * double(W) * double(H);
             if (px < 0.5) continue;
             const int res = clampi(int(std::lround(std::sqrt(px) * 1.0)), cfg.min_tex_res, cfg.max_tex_res);
@@ -445,15 +437,29 @@ Status plan_frame(std::vector<HTri>& scene, const aurora_patch_config& cfg, P3 e
     if (n_threads <= 1) {
         for (size_t i = 0; i < items.size(); ++i) run(i);
     } else {
+        // largest projected area first: the big subtrees start early, so
+        // the last ones to finish are small (each result lands at its own
+        // index, so the order of work changes no numbering)
+        std::vector<size_t> order(items.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return items[a].px > items[b].px; });
         std::atomic<size_t> next{0};
         std::vector<std::thread> pool;
         for (size_t w = 0; w < n_threads; ++w)
             pool.emplace_back([&] {
-                for (size_t i = next++; i < items.size(); i = next++) run(i);
+                for (size_t k = next++; k < order.size(); k = next++) run(order[k]);
             });
         for (std::thread& t : pool) t.join();
     }
-    for (const Plan& p : sub) AUR_TRY(p.status);
+    size_t n_texs = 0, n_warps = 0;
+    for (const Plan& p : sub) {
+        AUR_TRY(p.status);
+        n_texs += p.texs.size();
+        n_warps += p.warps.size();
+    }
+    out.texs.reserve(n_texs);
+    out.tex_depth.reserve(n_texs);
+    out.warps.reserve(n_warps + 2 * items.size() * 8);  // and the viewports' fans
     std::vector<uint32_t> root(items.size());
     for (int k = 0; k < 2; ++k) {
         for (size_t i = 0; i < items.size(); ++i) {
@@ -689,7 +695,9 @@ Status patch_render_device(PatchWork& pw, const aurora_patch_triangle* tris, uin
     const P3 eye{cam->origin[0], cam->origin[1], cam->origin[2]};
     const int W = int(cam->width), H = int(cam->height);
     const HTri vps[2] = {host_tri(cam->viewport[0], nullptr), host_tri(cam->viewport[1], nullptr)};
-    FramePlan plan;
+    // one plan per calling thread, kept between frames (see reset)
+    static thread_local FramePlan plan;
+    plan.reset();
     const Status planned = plan_frame(scene, *cfg, eye, vps, W, H, plan);
     RootArgs ra{};
     for (int k = 0; k < 2; ++k) {
