@@ -167,6 +167,84 @@ __device__ __forceinline__ void traverse_closest_wide(const DevScene& sc, v3 o, 
     prim_out = miss ? AURORA_MISS : (int)best_id;
 }
 
+// Resumable form of the four-wide walk: its state (reciprocal direction,
+// best hit, next node, stack depth) lives in the caller's registers and the
+// thread's LDS/spill stack column, and wide_walk_step runs one node step.  A
+// persistent kernel can then let the lanes whose walk ended shade and start
+// their next segment while the other lanes of the wave keep walking.  Node
+// order, tests and result are traverse_closest_wide's.
+struct WideWalk {
+    v3 inv;
+    float best_t;
+    uint32_t best_id;
+    uint32_t cur;  // the next node; kWalkDone once the walk has ended
+    int sp;
+};
+constexpr uint32_t kWalkDone = 0xffffffffu;
+
+__device__ __forceinline__ void wide_walk_start(WideWalk& w, v3 d, float tmax) {
+    w.inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    w.best_t = tmax;
+    w.best_id = 0x7fffffffu;
+    w.cur = 0;
+    w.sp = 0;
+}
+
+// one node step of the walk; true once it has ended
+__device__ __forceinline__ bool wide_walk_step(WideWalk& w, const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
+                                               uint32_t* stk, int col_stride, uint32_t* spill, size_t sstride) {
+    const uint32_t n_spheres = sc.n_spheres;
+    const uint32_t n_prims = sc.n_spheres + sc.n_quads;
+    // a well-formed tree only refers to nodes below n_wide; any other value
+    // ends the ray instead of reading out of bounds
+    if (w.cur >= sc.n_wide) {
+        w.cur = kWalkDone;
+        return true;
+    }
+    auto test_prim = [&](uint32_t id, float tn) {
+        const bool valid = id < n_prims;
+        const uint32_t k = valid ? id : 0u;
+        float t = 0.0f;
+        bool h;
+        if (k < n_spheres) {
+            h = hit_sphere_t(ldg4(sc.spheres, k), o, d, tmin, tmax, t);
+        } else {
+            const size_t q = 3 * (size_t)(k - n_spheres);
+            h = hit_quad_t(ldg4(sc.quads, q), ldg4(sc.quads, q + 1), ldg4(sc.quads, q + 2), o, d, tmin, tmax, t);
+        }
+        const bool take = valid & h & (t >= tn) & ((t < w.best_t) | ((t == w.best_t) & (id < w.best_id)));
+        w.best_t = take ? t : w.best_t;
+        w.best_id = take ? id : w.best_id;
+    };
+    uint32_t r0, r1, r2, r3;
+    const int m = wide_step(sc.wide, w.cur, o, w.inv, tmin, w.best_t, test_prim, r0, r1, r2, r3);
+    if (m == 0) {
+        if (w.sp == 0) {
+            w.cur = kWalkDone;
+            return true;
+        }
+        --w.sp;
+        w.cur = w.sp < kTravLds ? stk[w.sp * col_stride] : ldg1(spill, (size_t)(w.sp - kTravLds) * sstride);
+        return false;
+    }
+    auto push = [&](uint32_t v) {
+        if (w.sp < kTravLds) stk[w.sp * col_stride] = v;
+        else if (w.sp < kTravMaxStack) spill[(size_t)(w.sp - kTravLds) * sstride] = v;
+        if (w.sp < kTravMaxStack) ++w.sp;
+    };
+    if (m > 3) push(r3);
+    if (m > 2) push(r2);
+    if (m > 1) push(r1);
+    w.cur = r0;
+    return false;
+}
+
+__device__ __forceinline__ void wide_walk_result(const WideWalk& w, float tmax, float& t_out, int& prim_out) {
+    const bool miss = w.best_id == 0x7fffffffu;
+    t_out = miss ? tmax : w.best_t;
+    prim_out = miss ? AURORA_MISS : (int)w.best_id;
+}
+
 // Flat form, for scenes of a few primitives: every leaf record in the flat
 // array's order (largest boxes first, so the best t falls early and more
 // boxes are culled), its box's slab test against the current best t and its
