@@ -106,73 +106,15 @@ __device__ __forceinline__ int wide_step(const float4* W, uint32_t cur, v3 o, v3
     return m;
 }
 
-// Single-ray form of the four-wide traversal, for kernels that trace a ray
-// to completion in one thread.  stk: this thread's LDS column (level k at
-// stk[k * col_stride]); spill: its global column for levels >= kTravLds.
-__device__ __forceinline__ void traverse_closest_wide(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
-                                                      uint32_t* stk, int col_stride, uint32_t* spill, size_t sstride,
-                                                      float& t_out, int& prim_out) {
-    const float4* const W = sc.wide;
-    const float4* const S = sc.spheres;
-    const float4* const Qd = sc.quads;
-    const uint32_t n_spheres = sc.n_spheres, n_wide = sc.n_wide;
-    const uint32_t n_prims = sc.n_spheres + sc.n_quads;
-    const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    float best_t = tmax;
-    uint32_t best_id = 0x7fffffffu;
-    // tn: the entry distance of the primitive's box; hits before it do not count
-    // (an id past the primitives, which no well-formed tree holds, reads
-    // primitive 0 and is never taken)
-    auto test_prim = [&](uint32_t id, float tn) {
-        const bool valid = id < n_prims;
-        const uint32_t k = valid ? id : 0u;
-        float t = 0.0f;
-        bool h;
-        if (k < n_spheres) {
-            h = hit_sphere_t(ldg4(S, k), o, d, tmin, tmax, t);
-        } else {
-            const size_t q = 3 * (size_t)(k - n_spheres);
-            h = hit_quad_t(ldg4(Qd, q), ldg4(Qd, q + 1), ldg4(Qd, q + 2), o, d, tmin, tmax, t);
-        }
-        const bool take = valid & h & (t >= tn) & ((t < best_t) | ((t == best_t) & (id < best_id)));
-        best_t = take ? t : best_t;
-        best_id = take ? id : best_id;
-    };
-    uint32_t cur = 0;
-    int sp = 0;
-    auto push = [&](uint32_t v) {
-        if (sp < kTravLds) stk[sp * col_stride] = v;
-        else if (sp < kTravMaxStack) spill[(size_t)(sp - kTravLds) * sstride] = v;
-        if (sp < kTravMaxStack) ++sp;
-    };
-    // a well-formed tree only refers to nodes below n_wide; any other value
-    // ends the ray instead of reading out of bounds
-    while (cur < n_wide) {
-        uint32_t r0, r1, r2, r3;
-        const int m = wide_step(W, cur, o, inv, tmin, best_t, test_prim, r0, r1, r2, r3);
-        if (m == 0) {
-            if (sp == 0) break;
-            --sp;
-            if (sp < kTravLds) cur = stk[sp * col_stride];
-            else cur = ldg1(spill, (size_t)(sp - kTravLds) * sstride);
-            continue;
-        }
-        if (m > 3) push(r3);
-        if (m > 2) push(r2);
-        if (m > 1) push(r1);
-        cur = r0;
-    }
-    const bool miss = best_id == 0x7fffffffu;
-    t_out = miss ? tmax : best_t;
-    prim_out = miss ? AURORA_MISS : (int)best_id;
-}
-
-// Resumable form of the four-wide walk: its state (reciprocal direction,
-// best hit, next node, stack depth) lives in the caller's registers and the
-// thread's LDS/spill stack column, and wide_walk_step runs one node step.  A
-// persistent kernel can then let the lanes whose walk ended shade and start
-// their next segment while the other lanes of the wave keep walking.  Node
-// order, tests and result are traverse_closest_wide's.
+// The four-wide walk of one ray, resumable: its state (reciprocal
+// direction, best hit, next node, stack depth) lives in the caller's
+// registers and the thread's stack column (level k at stk[k * col_stride]
+// in LDS, at spill[(k - kTravLds) * sstride] from kTravLds on), and
+// wide_walk_step runs one node step: wide_step at the current node, then
+// the nearest hit child next and the others pushed farthest first, or a pop
+// when no child is hit.  A persistent kernel can so let the lanes whose walk
+// ended shade and start their next segment while the other lanes of the
+// wave keep walking.
 struct WideWalk {
     v3 inv;
     float best_t;
