@@ -328,7 +328,7 @@ def cornell_triangles():
     return b.arrays()
 
 
-def deep_chain_scene(levels=30, cluster=4096):
+def deep_chain_scene(levels=30, cluster=4096, glass=False):
     """A degenerate LBVH: a spine of `levels` binary levels, one per Morton
     bit (from the top), each peeling off a group of three coincident spheres
     whose box is larger than everything below, then `cluster` coincident tiny
@@ -336,19 +336,24 @@ def deep_chain_scene(levels=30, cluster=4096):
     so group k sits in the cell whose Morton code has only bit 29-k set.  The
     greedy four-wide collapse opens each group before the spine, so it
     advances one binary level per wide level and its walk's worst-case
-    stack need (three pushes per level) exceeds 96 entries."""
+    stack need (three pushes per level) exceeds 96 entries.  glass: the
+    groups alternate glass (ior 1.5) and grey diffuse and the cluster emits,
+    so that rendered paths enter the nested spheres and walk deep into the
+    tree (same geometry, same tree)."""
     b = Builder()
     m = b.lambertian((0.5, 0.5, 0.5))
-    b.sphere((1.0, 1.0, 1.0), 1e-4, m)
+    g = b.dielectric(1.5) if glass else m
+    e = b.light((4.0, 3.0, 2.0)) if glass else m
+    b.sphere((1.0, 1.0, 1.0), 1e-4, e)
     for k in range(levels):
         axis, bit = k % 3, 9 - k // 3
         c = [0.5 / 1024.0] * 3
         c[axis] = (2.0 ** bit + 0.5) / 1024.0
         r = 2.0 * 2.0 ** (-k / 3.0)
         for _ in range(3):
-            b.sphere(tuple(c), r, m)
+            b.sphere(tuple(c), r, g if k % 2 == 0 else m)
     for _ in range(cluster):
-        b.sphere((0.0, 0.0, 0.0), 1e-4, m)
+        b.sphere((0.0, 0.0, 0.0), 1e-4, e)
     return b.arrays()
 
 

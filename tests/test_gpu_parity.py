@@ -338,6 +338,33 @@ def test_flat_walk_sizes_bit_exact(ctx, ns, nt):
     assert segs > cam.image_width * cam.image_height * rs.spp  # some paths bounce
 
 
+def test_render_deep_tree_bit_exact(ctx):
+    """Paths through nested glass spheres over a tree 21 four-wide levels
+    deep (the pairwise collapse: three pushes per level reach the stack's
+    global spill past its 16 LDS levels): radiance and segment counts of the
+    resumable four-wide walk, the binary walk and the wavefront kernels
+    against the oracle."""
+    say("render deep tree")
+    arr = scenes.deep_chain_scene(glass=True)
+    cam = Camera(32, 24, 40.0, (1.5, 1.2, 2.0), (0.0, 0.0, 0.0))
+    rs = RenderSettings(spp=4, max_depth=16, background=(0.7, 0.8, 1.0), tmin=1e-4)
+    g, gs, o, os_ = both(ctx, arr)
+    assert gs["wide_depth"] > 16
+    orgb, _, _, segs = o.render(cam, rs)
+    assert len(np.unique(orgb.reshape(-1, 3), axis=0)) > 20
+    for tail, tree in ((1 << 30, 1), (1 << 30, 0), (2048, 1), (0, 1)):
+        ctx.set_launch(tail_paths=tail)
+        ctx.set_render_tree(tree)
+        try:
+            grgb, _, _, st = g.render(cam, rs)
+        finally:
+            ctx.set_launch()
+            ctx.set_render_tree(0)
+        say("deep tree tail", tail, "tree", tree, "segments gpu/cpu", st["segments"], segs)
+        assert st["segments"] == segs
+        assert np.array_equal(grgb.view(np.uint32), orgb.view(np.uint32))
+
+
 def test_render_textures_bit_exact(ctx):
     """Image textures on a sphere (spherical u, v from the series atan2 and
     acos), a parallelogram, a triangle and an emitter, with checker, Perlin,
