@@ -204,6 +204,8 @@ def spheres_section(ctx, n_pixels=64):
     x, r = g.astype(np.float64) / 500, f64 / 500
     scene.close()
     return {"workload": "random_spheres_1200x675_500spp_depth50", "paths": st["paths"], "seconds": wall,
+            "render_tree": {0: "binary", 1: "four-wide"}[tree],
+            "tree_trial_s_16spp": {{0: "binary", 1: "four-wide"}[k]: v for k, v in trial.items()},
             "paths_per_s": st["paths"] / wall, "rays_per_s": st["segments"] / wall,
             "render_tree": ["binary", "four-wide"][tree], "tree_trial_s": {str(k): v for k, v in trial.items()},
             "fp64_check": {"pixels": n_pixels, "spp": 500,
