@@ -6,7 +6,7 @@ sizes against the C compiler's (tests/test_abi.py).
 import ctypes as C
 import os
 
-ABI_VERSION = 5  # AURORA_ABI_VERSION of the header this mirror follows
+ABI_VERSION = 6  # AURORA_ABI_VERSION of the header this mirror follows
 
 AURORA_OK = 0
 AURORA_ERR_INVALID = 1
@@ -211,6 +211,7 @@ EXPORTS = [
     ("aurora_denoiser_train_step", C.c_int,
      [_vp, _vp, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _f, C.POINTER(C.c_float)]),
     ("aurora_selftest_mfma", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("aurora_selftest_rcp", C.c_int, [_vp, C.c_uint32, C.c_uint32, _vp, _vp]),
 ]
 
 _lib = None

@@ -187,6 +187,14 @@ class Context:
         the images are the same."""
         self.check(self.lib.aurora_set_render_tree(self.h, int(tree)))
 
+    def selftest_rcp(self, lo, hi):
+        """(mismatches, first mismatching bit patterns) of the kernels' fast
+        reciprocal against IEEE division over float32 bit patterns [lo, hi)."""
+        n = C.c_uint64(0)
+        first = (C.c_uint32 * 16)()
+        self.check(self.lib.aurora_selftest_rcp(self.h, int(lo), int(hi), C.byref(n), first))
+        return int(n.value), [int(first[k]) for k in range(min(16, int(n.value)))]
+
     def set_queue(self, pass_paths=0, claim_paths=0):
         """Queue sizes: most paths per automatic pass (0: 2^27) and paths per
         work-pool claim in the fused tail (0: by pass size).  Speed only."""

@@ -233,6 +233,31 @@ struct DevScene {
     uint32_t n_spheres, n_quads, n_nodes, n_wide, n_flat;
 };
 
+// 1/x rounded as IEEE division rounds it, in 5 VALU instructions instead of
+// the 11 of the division sequence: the hardware reciprocal (within one ulp)
+// refined by one Newton-Raphson step on fused multiply-adds, e = 1 - x r,
+// r' = r + r e (P. Markstein, IA-64 and Elementary Functions, 2000).  For
+// every x with 2^-126 <= |x| < 2^126 the fast form equals 1.0f / x bit for
+// bit: aurora_selftest_rcp compares all of them on the device (the sign is
+// symmetric).  Other x (zero, denormal, huge, infinite or NaN), whose
+// quotient may be denormal or special, take the division.
+__device__ __forceinline__ float rcp_fast(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+}
+__device__ __forceinline__ float rcp_rn(float x) {
+    float y = rcp_fast(x);
+    const float ax = fabsf(x);
+    if (!(ax >= 0x1p-126f && ax < 0x1p126f)) {
+        // an empty volatile statement keeps the division behind the branch
+        // (lanes with a normal x skip it); otherwise the compiler runs it for
+        // every lane and selects
+        __asm__ volatile("");
+        y = 1.0f / x;
+    }
+    return y;
+}
+
 // Ray/sphere roots without the cancellations of the textbook quadratic
 // (Haines, Guenther and Akenine-Moeller, "Precision Improvements for
 // Ray/Sphere Intersection", Ray Tracing Gems, 2019): the discriminant
@@ -296,7 +321,7 @@ __device__ __forceinline__ bool hit_quad(float4 q0, float4 q1, float4 q2, v3 o, 
     v3 h = cross(d, v);
     float a = dot(u, h);
     if (fabsf(a) < 1e-20f) return false;
-    float f = 1.0f / a;
+    float f = rcp_rn(a);
     v3 s = o - xyz(q0);
     float alpha = f * dot(s, h);
     if (alpha < 0.0f || alpha > 1.0f) return false;
@@ -323,7 +348,7 @@ __device__ __forceinline__ bool hit_quad_t(float4 q0, float4 q1, float4 q2, v3 o
     const v3 u = xyz(q1), v = xyz(q2);
     const v3 h = cross(d, v);
     const float a = dot(u, h);
-    const float f = 1.0f / a;
+    const float f = rcp_rn(a);
     const v3 s = o - xyz(q0);
     const float alpha = f * dot(s, h);
     const v3 qv = cross(s, u);
@@ -454,7 +479,7 @@ __device__ __forceinline__ v3 tex_leaf(const DevScene& sc, const aurora_texture&
 __device__ __forceinline__ v3 texture_value(const DevScene& sc, uint32_t ti, float u, float v, v3 p) {
     aurora_texture t = sc.texs[ti];
     if (t.type == AURORA_TEX_CHECKER) {
-        float inv = 1.0f / t.scale;
+        float inv = rcp_rn(t.scale);
         int cx = (int)fminf(fmaxf(floorf(inv * p.x), -1073741824.0f), 1073741824.0f);
         int cy = (int)fminf(fmaxf(floorf(inv * p.y), -1073741824.0f), 1073741824.0f);
         int cz = (int)fminf(fmaxf(floorf(inv * p.z), -1073741824.0f), 1073741824.0f);

@@ -272,7 +272,7 @@ __device__ __forceinline__ ShadeOut shade_core(const SceneRec& r, const ShadeBuf
         if (dot(rf, n) <= 0.0f) scatter = false;  // absorbed
         else nd = unitv(rf);
     } else {
-        const float ri = front ? 1.0f / m.ior : m.ior;
+        const float ri = front ? rcp_rn(m.ior) : m.ior;
         const float cos_t = fminf(dot(-dir, n), 1.0f);
         const float sin_t = sqrtf(1.0f - cos_t * cos_t);
         const bool cannot = ri * sin_t > 1.0f;

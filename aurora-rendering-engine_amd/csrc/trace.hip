@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(ExtendArgs a) {
                         n_nodes = WIDE ? rec.sc.n_wide : rec.sc.n_nodes;
                         n_prims = rec.sc.n_spheres + rec.sc.n_quads;
                     }
-                    inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                    inv = V(rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z));
                     best_t = tmax;
                     best_id = kNoPrim;
                     sp = 0;
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_brute(DevScene sc, const float4* ro,
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const v3 o = xyz(ro[i]), d = xyz(rd[i]);
-    const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const v3 inv = V(rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z));
     float best_t = tmax;
     uint32_t best_id = kNoPrim;
     const uint32_t np = sc.n_spheres + sc.n_quads;

@@ -125,7 +125,7 @@ struct WideWalk {
 constexpr uint32_t kWalkDone = 0xffffffffu;
 
 __device__ __forceinline__ void wide_walk_start(WideWalk& w, v3 d, float tmax) {
-    w.inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    w.inv = V(rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z));
     w.best_t = tmax;
     w.best_id = 0x7fffffffu;
     w.cur = 0;
@@ -207,7 +207,7 @@ __device__ __forceinline__ void traverse_closest_flat(const DevScene& sc, v3 o, 
     const uint32_t n_spheres = __builtin_amdgcn_readfirstlane(sc.n_spheres);
     const uint32_t n_flat = __builtin_amdgcn_readfirstlane(sc.n_flat);
     const uint32_t n_prims = n_spheres + __builtin_amdgcn_readfirstlane(sc.n_quads);
-    const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const v3 inv = V(rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z));
     float best_t = tmax;
     uint32_t best_id = 0x7fffffffu;
     for (uint32_t k = 0; k < n_flat; ++k) {
@@ -251,7 +251,7 @@ __device__ __forceinline__ void traverse_closest_binary(const DevScene& sc, v3 o
     const float4* const Qd = sc.quads;
     const uint32_t n_spheres = sc.n_spheres, n_nodes = sc.n_nodes;
     const uint32_t n_prims = sc.n_spheres + sc.n_quads;
-    const v3 inv = V(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const v3 inv = V(rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z));
     float best_t = tmax;
     uint32_t best_id = 0x7fffffffu;
     // tn: the entry distance of the primitive's box; hits before it do not count

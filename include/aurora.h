@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define AURORA_ABI_VERSION 5
+#define AURORA_ABI_VERSION 6
 
 /* ---- status codes ---------------------------------------------------- */
 #define AURORA_OK 0
@@ -578,6 +578,15 @@ int aurora_denoiser_train_step(aurora_denoiser* dn, const float* d_rgb_sum,
  * row-major) x B (32x16). Host buffers. */
 int aurora_selftest_mfma(aurora_ctx* ctx, const uint16_t* A, const uint16_t* B,
                          float* D);
+/* Self-test of the fast reciprocal the path kernels use: the float32 bit
+ * patterns in [lo, hi) whose hardware-reciprocal-plus-Newton-step result
+ * differs from IEEE division 1.0f / x are counted into *mismatches, and the
+ * first (up to 16, in no particular order) written to first (may be NULL).
+ * The kernels use that form only for 2^-126 <= |x| < 2^126, the patterns
+ * [0x00800000, 0x7e800000) and [0x80800000, 0xfe800000), where the count
+ * is zero.  No counterpart in the reference (its divisions run on the CPU). */
+int aurora_selftest_rcp(aurora_ctx* ctx, uint32_t lo, uint32_t hi,
+                        uint64_t* mismatches, uint32_t* first);
 
 #ifdef __cplusplus
 }
