@@ -26,6 +26,12 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// The number of lanes below this one whose bit is set in mask (v_mbcnt:
+// a ballot rank without a per-lane mask register).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 // The scene record of a path.  A single-scene launch reads recs[0] through
 // the constant address space: nothing writes the table while a render
 // kernel runs, so its fields arrive by scalar loads and the scene's pointers
@@ -309,8 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t
     uint32_t* next_count = &b.counters[b.cur ^ 1];
     if (blockIdx.x == 0 && threadIdx.x < kPools) b.counters[kPoolBase + kPoolStride * threadIdx.x] = 0u;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    unsigned long long segs = 0;
+    uint32_t segs = 0;  // segments this lane traced (a few hundred per launch)
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         const uint32_t i = base + tid;
         ShadeOut o;
@@ -340,16 +345,17 @@ __global__ __launch_bounds__(kBlock) void k_shade(Pass pa, ShadeBufs b, uint32_t
         if (tid == 0) s_base = total ? atomicAdd(next_count, total) : 0u;
         __syncthreads();
         if (o.alive) {
-            const uint32_t slot = s_base + before + (uint32_t)__popcll(m & lt);
+            const uint32_t slot = s_base + before + lanes_below(m);
             b.no[slot] = make_float4(o.p.x, o.p.y, o.p.z, __uint_as_float(path));
             b.nd[slot] = make_float4(o.nd.x, o.nd.y, o.nd.z, __uint_as_float(sid));
             b.nb[slot] = make_float4(o.beta.x, o.beta.y, o.beta.z, 0.0f);
         }
         __syncthreads();  // s_wave / s_base are reused by the next iteration
     }
+    unsigned long long wsegs = segs;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) segs += __shfl_xor(segs, off);
-    if (lane == 0 && segs) atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kTailSegs), segs);
+    for (int off = 32; off >= 1; off >>= 1) wsegs += __shfl_xor(wsegs, off);
+    if (lane == 0 && wsegs) atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kTailSegs), wsegs);
 }
 
 // Fused tail: once the live queue is short, each lane carries one path to
@@ -385,7 +391,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 
     const uint32_t n = b.counters[b.cur];
     uint32_t* my_spill = spill + (size_t)blockIdx.x * kBlock + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // the queue is split into one contiguous range per XCD; a wave whose
     // range is spent moves on to the next ones, so uneven ranges (a batch's
     // scenes differ in cost per path) still finish together
@@ -400,7 +405,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 
     bool has = false, done = false;
     uint32_t path = 0, sid = 0, depth = 0, pixel = 0, sample = 0;  // pixel, sample: once per path
     v3 org = V(0, 0, 0), dir = V(0, 0, 0), beta = V(0, 0, 0);
-    unsigned long long segs = 0;
+    uint32_t segs = 0;  // segments this lane traced (a few hundred per launch)
     WideWalk walk{};       // TREE 1: the lane's walk of its current segment
     bool walking = false;  // TREE 1: that walk has not ended
     for (;;) {
@@ -430,7 +435,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 
                 }
             }
             const uint32_t avail = b_next < b_end ? b_end - b_next : 0u;
-            const uint32_t rank = (uint32_t)__popcll(im & lt);
+            const uint32_t rank = lanes_below(im);
             if (idle) {
                 if (avail == 0) {
                     done = true;
@@ -496,9 +501,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 
             }
         }
     }
+    unsigned long long wsegs = segs;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) segs += __shfl_xor(segs, off);
-    if (lane == 0 && segs) atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kTailSegs), segs);
+    for (int off = 32; off >= 1; off >>= 1) wsegs += __shfl_xor(wsegs, off);
+    if (lane == 0 && wsegs) atomicAdd(reinterpret_cast<unsigned long long*>(b.counters + kTailSegs), wsegs);
 }
 
 // per pixel: add the pass's samples in sample order to the scene's sums
