@@ -25,8 +25,10 @@ using namespace aur;
 // at least this many nodes (32 bytes each: 64 MiB, beyond what the L2s hold)
 constexpr uint32_t kSortMinRays = 1u << 20;
 // resident 256-thread blocks of the fused tail kernel per CU: it is built for
-// five waves per SIMD for the tree walks, eight for the flat walk
+// five waves per SIMD for the tree walks, six for the four-wide walk of a
+// single scene, eight for the flat walk
 constexpr int kTailBlocksPerCu = 5;
+constexpr int kTailWideOneBlocksPerCu = 6;
 constexpr int kTailFlatBlocksPerCu = 8;
 constexpr uint32_t kSortMinNodes = 1u << 21;
 
@@ -538,7 +540,9 @@ Status render_jobs(aurora_ctx* c, std::vector<Job>& jobs, aurora_render_stats* s
     // the passes walk the flat leaf array under the condition render_jobs
     // sets pa.flat by (one scene, small enough to have the array)
     const bool flat = c->render_tree == 2 && jobs.size() == 1 && jobs[0].scene->view.n_flat > 0;
-    const int tail_grid = c->cus * (flat ? kTailFlatBlocksPerCu : kTailBlocksPerCu);
+    const bool wide_one = !flat && c->render_tree != 0 && jobs.size() == 1;
+    const int tail_grid =
+        c->cus * (flat ? kTailFlatBlocksPerCu : wide_one ? kTailWideOneBlocksPerCu : kTailBlocksPerCu);
     const int graph_tail_grid = tail_grid;
     AUR_TRY(ensure_workspace(c, (uint32_t)cap, std::max(ext_grid, graph ? graph_tail_grid : tail_grid)));
     Workspace& w = c->ws;

@@ -384,7 +384,7 @@ constexpr uint32_t kTailBatchMin = 64, kTailBatchMax = 128;
 // the binary one, 2 (single-scene passes of small scenes) the flat leaf array
 // (the faster form depends on the scene, so the launch shape chooses).
 template <bool ONE, int TREE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 2 ? 8 : 5))) void k_tail(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 2 ? 8 : (ONE && TREE == 1) ? 6 : 5))) void k_tail(
     Pass pa, ShadeBufs b, uint32_t depth0, uint32_t* spill,
                                                  uint32_t spill_stride) {
     __shared__ uint32_t stk[kTravLds * kBlock];
