@@ -636,7 +636,7 @@ def main():
                          "(calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               "(the render would shard by image tiles or sample ranges with one final gather)", file=sys.stderr)
         sys.exit(2)
 

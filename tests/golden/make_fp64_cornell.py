@@ -1,8 +1,8 @@
 """Writes tests/golden/cornell64_fp64_131072spp.npz: the Cornell box at
 64x64 (depth 50, Philox seed 0, background black) rendered by the CPU
 oracle in FP64 at 2^17 = 131072 samples per pixel (samples [0, 131072)),
-the "FP64 CPU render converged at 10^5 spp" of the north star at a size the
-host finishes in minutes.  Stored: the per-pixel FP64 radiance sums and the
+an FP64 CPU render converged past 10^5 spp (the reference image of the
+time-to-relMSE metric) at a size the host finishes in minutes.  Stored: the per-pixel FP64 radiance sums and the
 segment count.  GPU tests render the same samples in FP32 and compare.
 
 Run from the repository root:  python tests/golden/make_fp64_cornell.py

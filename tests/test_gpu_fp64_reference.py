@@ -10,7 +10,7 @@ FP32 paths end slightly sooner (0.08% fewer segments) and the image mean is
 the rest behaves like extra sampling noise and shrinks as 1/spp.  The bounds
 here, relMSE below 1e-5 (100x below the 1e-3 of the time-to-relMSE target),
 the mean within 0.2% and the segment count within 0.2%, are what make an
-FP32 GPU render of at least 2^17 spp a stand-in for the FP64 CPU judge."""
+FP32 GPU render of at least 2^17 spp a stand-in for the FP64 CPU reference."""
 import os
 
 import numpy as np
@@ -29,7 +29,7 @@ def test_fp32_gpu_render_matches_fp64_oracle_at_2_17_spp(ctx):
     g = np.load(GOLDEN)
     spp = int(g["spp"])
     ref = g["rgb_sum"] / spp
-    print("fp64 judge: rendering", spp, "spp at 64x64", flush=True)
+    print("fp64 reference: rendering", spp, "spp at 64x64", flush=True)
     sc = ctx.upload(scenes.cornell_box())
     sc.build_bvh()
     rgb = torch.zeros((64, 64, 3), dtype=torch.float32, device="cuda")
@@ -37,7 +37,7 @@ def test_fp32_gpu_render_matches_fp64_oracle_at_2_17_spp(ctx):
     x = rgb.cpu().numpy().astype(np.float64) / spp
     rel = float(np.mean((x - ref) ** 2 / (ref ** 2 + 0.01)))
     seg_ratio = st["segments"] / float(g["segments"])
-    print(f"fp64 judge: relMSE(FP32 GPU, FP64 oracle) = {rel:.3e}, mean {x.mean():.6f} vs {ref.mean():.6f}, "
+    print(f"fp64 reference: relMSE(FP32 GPU, FP64 oracle) = {rel:.3e}, mean {x.mean():.6f} vs {ref.mean():.6f}, "
           f"segments ratio {seg_ratio:.6f}", flush=True)
     assert rel < 1e-5
     assert abs(x.mean() - ref.mean()) < 2e-3 * ref.mean()
