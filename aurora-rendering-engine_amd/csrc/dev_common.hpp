@@ -257,6 +257,17 @@ __device__ __forceinline__ float rcp_rn(float x) {
     }
     return y;
 }
+// rcp_rn for a caller that discards the quotient when |x| < 1e-20 (the quad
+// test's determinant): below 2^-126 any value will do, so only the top of the
+// verified range is checked.  Equal to rcp_rn for every |x| >= 2^-126 and NaN.
+__device__ __forceinline__ float rcp_rn_unless_tiny(float x) {
+    float y = rcp_fast(x);
+    if (!(fabsf(x) < 0x1p126f)) {
+        __asm__ volatile("");
+        y = 1.0f / x;
+    }
+    return y;
+}
 
 // Ray/sphere roots without the cancellations of the textbook quadratic
 // (Haines, Guenther and Akenine-Moeller, "Precision Improvements for
@@ -348,7 +359,9 @@ __device__ __forceinline__ bool hit_quad_t(float4 q0, float4 q1, float4 q2, v3 o
     const v3 u = xyz(q1), v = xyz(q2);
     const v3 h = cross(d, v);
     const float a = dot(u, h);
-    const float f = rcp_rn(a);
+    // a quotient for |a| < 1e-20 is never used: the test rejects those and t
+    // counts only on a hit
+    const float f = rcp_rn_unless_tiny(a);
     const v3 s = o - xyz(q0);
     const float alpha = f * dot(s, h);
     const v3 qv = cross(s, u);
@@ -370,6 +383,26 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, v3 o, v3 inv, float t
     float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tcur));
     tnear = tn;
     return tn <= tf;
+}
+
+// slab() for a walk whose tcur is never NaN (it starts at a tmax that is not
+// NaN and only takes hit distances): the same decision and tnear.  tcur
+// enters by a compare instead of a min, so no NaN-quieting max is issued on
+// it per box.  For tn not NaN, tn <= fminf(a, tcur) is !(tn > a) & (tn <= tcur)
+// whether or not a is NaN; for tn NaN both forms are false.
+__device__ __forceinline__ bool slab_tcur(float4 lo, float4 hi, v3 o, v3 inv, float tmin, float tcur, float& tnear) {
+    float tx0 = (lo.x - o.x) * inv.x, tx1 = (hi.x - o.x) * inv.x;
+    float ty0 = (lo.y - o.y) * inv.y, ty1 = (hi.y - o.y) * inv.y;
+    float tz0 = (lo.z - o.z) * inv.z, tz1 = (hi.z - o.z) * inv.z;
+    // fmaxf(min z, tmin) by one v_max_f32 with tmin from a scalar register: no
+    // input here is a signalling NaN (products, and a tmin the host made), so
+    // the instruction's IEEE-mode result is fmaxf's
+    float mzt;
+    __asm__("v_max_f32 %0, %1, %2" : "=v"(mzt) : "s"(tmin), "v"(fminf(tz0, tz1)));
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), mzt);
+    float a = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    tnear = tn;
+    return !(tn > a) & (tn <= tcur);
 }
 
 // the box of primitive id as the LBVH stores it in the primitive's leaf

@@ -207,13 +207,14 @@ __device__ __forceinline__ void traverse_closest_flat(const DevScene& sc, v3 o, 
     const uint32_t n_spheres = __builtin_amdgcn_readfirstlane(sc.n_spheres);
     const uint32_t n_flat = __builtin_amdgcn_readfirstlane(sc.n_flat);
     const uint32_t n_prims = n_spheres + __builtin_amdgcn_readfirstlane(sc.n_quads);
+    const float tmin_s = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(tmin)));
     const v3 inv = V(rcp_rn(d.x), rcp_rn(d.y), rcp_rn(d.z));
     float best_t = tmax;
     uint32_t best_id = 0x7fffffffu;
     for (uint32_t k = 0; k < n_flat; ++k) {
         const float4 lo = ldu4(F, 2 * (size_t)k), hi = ldu4(F, 2 * (size_t)k + 1);
         float tn = 0.0f;
-        const bool hb = slab(lo, hi, o, inv, tmin, best_t, tn);
+        const bool hb = slab_tcur(lo, hi, o, inv, tmin_s, best_t, tn);
         const uint32_t id = __float_as_uint(lo.w) & ~AURORA_NODE_LEAF;
         const bool valid = id < n_prims;
         const uint32_t q = valid ? id : 0u;
