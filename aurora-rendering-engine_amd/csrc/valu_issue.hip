@@ -53,12 +53,16 @@ enum Mix {
     ADD_S_ADD,  // the same beside v_add_f32 on vector registers only
     ADD_S_MAX,
     SUB_S_MUL,  // a slab test's (scalar box - origin) * reciprocal
+    MULLO,      // v_mul_lo_u32
+    MULHI_MULLO,
+    MAD_U64,    // v_mad_u64_u32: a Philox round's 32x32 -> 64-bit product
+    MAD_U64_FMA,
 };
 
 // one instruction on chain j; kind k of the mix's pair
 template <Mix M, int J>
 __device__ __forceinline__ void step(float (&a)[kChains], uint32_t (&x)[kChains], double (&d)[kChains], float b,
-                                     float c, uint64_t mask, uint64_t& cm) {
+                                     float c, uint64_t mask, uint64_t& cm, uint64_t (&w)[kChains]) {
     constexpr bool even = (J % 2) == 0;
     // a wave-uniform value in a scalar register
     const float sb = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(uint32_t, b)));
@@ -95,6 +99,15 @@ __device__ __forceinline__ void step(float (&a)[kChains], uint32_t (&x)[kChains]
         asm volatile("v_xor_b32 %0, %0, %1" : "+v"(x[J]) : "v"(x[(J + 1) % kChains]));
     } else if constexpr (M == MULHI_MAX) {
         asm volatile("v_mul_hi_u32 %0, %0, %1" : "+v"(x[J]) : "v"(x[(J + 1) % kChains]));
+    } else if constexpr (M == MULLO || (M == MULHI_MULLO && !even)) {
+        asm volatile("v_mul_lo_u32 %0, %0, %1" : "+v"(x[J]) : "v"(x[(J + 1) % kChains]));
+    } else if constexpr (M == MULHI_MULLO) {
+        asm volatile("v_mul_hi_u32 %0, %0, %1" : "+v"(x[J]) : "v"(x[(J + 1) % kChains]));
+    } else if constexpr (M == MAD_U64 || (M == MAD_U64_FMA && even)) {
+        uint64_t carry;
+        asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(w[J]), "=s"(carry) : "v"(x[J]), "v"(x[(J + 1) % kChains]));
+    } else if constexpr (M == MAD_U64_FMA) {
+        asm volatile("v_fma_f32 %0, %0, %1, %2" : "+v"(a[J]) : "v"(b), "v"(c));
     } else if constexpr (M == FMA_F64) {
         const double db = b, dc = c;
         asm volatile("v_fma_f64 %0, %0, %1, %2" : "+v"(d[J]) : "v"(db), "v"(dc));
@@ -103,9 +116,9 @@ __device__ __forceinline__ void step(float (&a)[kChains], uint32_t (&x)[kChains]
 
 template <Mix M, int J>
 __device__ __forceinline__ void chains(float (&a)[kChains], uint32_t (&x)[kChains], double (&d)[kChains], float b,
-                                       float c, uint64_t mask, uint64_t& cm) {
-    step<M, J>(a, x, d, b, c, mask, cm);
-    if constexpr (J + 1 < kChains) chains<M, J + 1>(a, x, d, b, c, mask, cm);
+                                       float c, uint64_t mask, uint64_t& cm, uint64_t (&w)[kChains]) {
+    step<M, J>(a, x, d, b, c, mask, cm, w);
+    if constexpr (J + 1 < kChains) chains<M, J + 1>(a, x, d, b, c, mask, cm, w);
 }
 
 template <Mix M>
@@ -113,19 +126,21 @@ __global__ __launch_bounds__(kBlock) void k_issue(float* out, float b, float c) 
     float a[kChains];
     uint32_t x[kChains];
     double d[kChains];
+    uint64_t w[kChains];
     for (int j = 0; j < kChains; ++j) {
         a[j] = (float)threadIdx.x * 1e-3f + (float)j;
         x[j] = threadIdx.x * 2654435761u + (uint32_t)j;
         d[j] = (double)a[j];
+        w[j] = x[j];
     }
     const uint64_t mask = 0x5555555555555555ull;
     uint64_t cm = 0;
     for (int it = 0; it < kIters; ++it) {
 #pragma unroll
-        for (int r = 0; r < kReps; ++r) chains<M, 0>(a, x, d, b, c, mask, cm);
+        for (int r = 0; r < kReps; ++r) chains<M, 0>(a, x, d, b, c, mask, cm, w);
     }
     float s = (float)(cm & 1u);
-    for (int j = 0; j < kChains; ++j) s += a[j] + (float)(x[j] & 255u) + (float)d[j];
+    for (int j = 0; j < kChains; ++j) s += a[j] + (float)(x[j] & 255u) + (float)d[j] + (float)(w[j] & 255u);
     out[(size_t)blockIdx.x * kBlock + threadIdx.x] = s;
 }
 
@@ -162,6 +177,10 @@ int main() {
         {"v_add_f32 (scalar) + v_add_f32", k_issue<ADD_S_ADD>},
         {"v_add_f32 (scalar) + v_max_f32", k_issue<ADD_S_MAX>},
         {"v_sub_f32 (scalar) + v_mul_f32", k_issue<SUB_S_MUL>},
+        {"v_mul_lo_u32", k_issue<MULLO>},
+        {"v_mul_hi_u32 + v_mul_lo_u32", k_issue<MULHI_MULLO>},
+        {"v_mad_u64_u32", k_issue<MAD_U64>},
+        {"v_mad_u64_u32 + v_fma_f32", k_issue<MAD_U64_FMA>},
     };
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
