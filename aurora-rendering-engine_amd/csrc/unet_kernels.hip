@@ -369,18 +369,26 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(ConvIn in, const u16* __res
             for (int t = 0; t < NT; ++t)
                 acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[t], fa[mt], acc[mt][t], 0, 0, 0);
     };
-    // two fragment sets in turn: chunk q + 1 loads while chunk q multiplies
+    // two fragment sets in turn: chunk q + 1 loads while chunk q multiplies.
+    // A trip takes two chunks and has no exit in its middle (an odd last
+    // chunk follows the loop): with an exit after each chunk the compiler
+    // copied all 16 * NT accumulators out of the MFMA registers and back on
+    // every trip.  kpad is a multiple of 32.
     bf16x8 a0[4], b0[NT], a1[4], b1[NT];
     load(a0, b0);
-    for (int k0 = 0;; k0 += 64) {
-        const bool more1 = k0 + 32 < kpad;
-        if (more1) load(a1, b1);
+    int rest = kpad / 32 - 1;  // chunks after the first
+    for (; rest >= 2; rest -= 2) {
+        load(a1, b1);
         mma(a0, b0);
-        if (!more1) break;
-        const bool more2 = k0 + 64 < kpad;
-        if (more2) load(a0, b0);
+        load(a0, b0);
         mma(a1, b1);
-        if (!more2) break;
+    }
+    if (rest == 1) {
+        load(a1, b1);
+        mma(a0, b0);
+        mma(a1, b1);
+    } else {
+        mma(a0, b0);
     }
     // the MFMAs computed each tile transposed (weights as the A operand),
     // so this lane holds channels co .. co+3 of pixel x: one 8-byte (BF16)
