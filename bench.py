@@ -275,9 +275,11 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=65536):
         return torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
 
     def render_into(rgb, spp, offset, alb=None, nrm=None):
+        # no per-kernel statistics (their HIP events and counter read-back
+        # are not part of producing the image); the synchronize ends the render
         t0 = time.perf_counter()
         scene.render_device(cam, RenderSettings(spp=spp, sample_offset=offset, max_depth=50, seed=0,
-                                                aov=alb is not None), rgb, alb, nrm)
+                                                aov=alb is not None), rgb, alb, nrm, stats=False)
         ctx.synchronize()
         return time.perf_counter() - t0
 
