@@ -292,6 +292,7 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=65536):
            "reference": f"2 x {ref_spp} spp GPU FP32 halves, per-pixel variance subtracted",
            "reference_noise_relmse": plain_noise(A, B)}
     acc = buf()
+    render_into(buf(), 1024, 0)  # untimed, as the denoised side's warm-up below
     done, t_plain, curve = 0, 0.0, []
     err = float("inf")
     # coarse checkpoints, then every 1024 spp where plain crosses 1e-3
@@ -311,6 +312,14 @@ def time_to_relmse_section(ctx, scene, cam, target=1e-3, ref_spp=65536):
         dcurve = []
         best = None
         for spp in (16, 24, 32, 40, 48, 56, 64, 96, 128, 160, 192, 224, 256, 384, 512, 1024, 2048, 4096):
+            # the same render and denoise once untimed, into buffers of their
+            # own: a single call of a few ms otherwise also times the engine
+            # clock's ramp after the host-side gaps between calls (the first
+            # of six back-to-back 32-spp renders takes 10.6 ms, the sixth 7.4)
+            w = [buf(), buf(), buf()]
+            render_into(w[0], spp, 0, w[1], w[2])
+            dn.denoise(w[0], w[1], w[2], spp)
+            del w
             rgb, alb, nrm = buf(), buf(), buf()
             tr = render_into(rgb, spp, 0, alb, nrm)
             t0 = time.perf_counter()
