@@ -389,7 +389,8 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, v3 o, v3 inv, float t
 // NaN and only takes hit distances): the same decision and tnear.  tcur
 // enters by a compare instead of a min, so no NaN-quieting max is issued on
 // it per box.  For tn not NaN, tn <= fminf(a, tcur) is !(tn > a) & (tn <= tcur)
-// whether or not a is NaN; for tn NaN both forms are false.
+// whether or not a is NaN; for tn NaN both forms are false.  tmin must be
+// wave-uniform (it is read from a scalar register) and not a signalling NaN.
 __device__ __forceinline__ bool slab_tcur(float4 lo, float4 hi, v3 o, v3 inv, float tmin, float tcur, float& tnear) {
     float tx0 = (lo.x - o.x) * inv.x, tx1 = (hi.x - o.x) * inv.x;
     float ty0 = (lo.y - o.y) * inv.y, ty1 = (hi.y - o.y) * inv.y;

@@ -197,7 +197,8 @@ __device__ __forceinline__ void wide_walk_result(const WideWalk& w, float tmax, 
 // the same conditions as in the tree walks (its leaf box passes the slab
 // test at the current best t, the hit lies at or beyond the box's entry
 // distance, nearer or an equal t with a lower id), so the closest hit is the
-// tree walks' one.
+// tree walks' one.  tmax must not be NaN (the render passes +inf), so the
+// best t never is; tmin is the launch's one scene's, so wave-uniform.
 __device__ __forceinline__ void traverse_closest_flat(const DevScene& sc, v3 o, v3 d, float tmin, float tmax,
                                                       float& t_out, int& prim_out) {
     // one scene per launch: the scene's fields are wave-uniform
