@@ -114,8 +114,12 @@ __global__ __launch_bounds__(kBlock) void k_raygen(Pass pa, float4* qo, float4* 
     const uint32_t i = pixel % r.cam.W, j = pixel / r.cam.W;
     v3 org, dir;
     camera_ray(r.cam, pixel, i, j, r.sample_base + s_local, r.seed, org, dir);
-    qo[p] = make_float4(org.x, org.y, org.z, __uint_as_float(p));
     qd[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(sid));
+    // for the fused tail a queue position is its path, and a pinhole
+    // camera's rays all leave its centre, which the tail reads from the
+    // scene record: only a thin lens needs its origins written
+    if (pa.lean_queue && !r.cam.defocus) return;
+    qo[p] = make_float4(org.x, org.y, org.z, __uint_as_float(p));
     if (pa.lean_queue) return;  // the fused tail sets throughput 1 and writes every L
     qb[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     L[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -441,14 +445,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(TREE == 
                     done = true;
                 } else if (rank < avail) {
                     const uint32_t i = r_lo + b_next + rank;
-                    const float4 O = b.qo[i], D = b.qd[i];
-                    path = __float_as_uint(O.w);
+                    const float4 D = b.qd[i];
                     sid = ONE ? 0u : __float_as_uint(D.w);
-                    org = xyz(O);
                     dir = xyz(D);
-                    beta = pa.lean_queue ? V(1.0f, 1.0f, 1.0f) : xyz(b.qb[i]);
+                    const SceneRec rc = scene_rec<ONE>(pa.recs, sid);
+                    if (pa.lean_queue) {
+                        // the raygen queue: position i is path i; a pinhole
+                        // camera's origin is its centre (k_raygen)
+                        path = i;
+                        org = rc.cam.defocus ? xyz(b.qo[i]) : rc.cam.center;
+                        beta = V(1.0f, 1.0f, 1.0f);
+                    } else {
+                        const float4 O = b.qo[i];
+                        path = __float_as_uint(O.w);
+                        org = xyz(O);
+                        beta = xyz(b.qb[i]);
+                    }
                     depth = depth0;
-                    path_pixel(scene_rec<ONE>(pa.recs, sid), path, pixel, sample);
+                    path_pixel(rc, path, pixel, sample);
                     has = true;
                     if constexpr (TREE == 1) {
                         wide_walk_start(walk, dir, INFINITY);
