@@ -228,9 +228,9 @@ TAIL_ALWAYS = (1 << 32) - 1  # fused-tail switch above any queue: one persistent
 # work every time, so these counts hold for every launch; the fractions below
 # divide them by the launch times measured live.
 PMC = {
-    ("k_tail", 1024, 32, TAIL_ALWAYS, 2): {"fetch_kib": 535887.0, "write_kib": 1072514.0,
-                                          "hbm_bytes": (535887.0 * 2.0 + 1072514.0) * 1024.0,
-                                          "valu_insts": 6.2985e9, "valu_util": 90.00, "valu_busy": 146.0},
+    ("k_tail", 1024, 32, TAIL_ALWAYS, 2): {"fetch_kib": 271892.0, "write_kib": 997197.0,
+                                          "hbm_bytes": (271892.0 * 2.0 + 997197.0) * 1024.0,
+                                          "valu_insts": 6.3474e9, "valu_util": 89.34, "valu_busy": 146.9},
 }
 # MI355X: 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock.  One wave64
 # VALU instruction of a kind occupies its SIMD for 4 cycles (614.4 G/s chip
