@@ -178,7 +178,8 @@ struct Pass {
     uint32_t flat;         // the fused tail walks the scene's flat leaf array (one scene with one)
     uint32_t claim;        // paths per work-pool claim in the fused tail (0: by pass size)
     uint32_t lean_queue;   // the fused tail carries the pass from the first bounce: raygen writes
-                           // origins and directions only (throughput 1, radiance set by the tail)
+                           // directions, and origins for a thin lens only (throughput 1, radiance
+                           // set by the tail; a queue position is its path)
 };
 struct Workspace {
     DevBuf q_o[2], q_d[2], q_b[2];   // path queues (float4 each)
