@@ -436,8 +436,8 @@ Status ensure_workspace(aurora_ctx* c, uint32_t cap, int grid) {
         }
         AUR_TRY(w.hit.alloc((size_t)cap * 8));
         AUR_TRY(w.L.alloc((size_t)cap * 16));
-        AUR_TRY(w.aov_a.alloc((size_t)cap * 16));
-        AUR_TRY(w.aov_n.alloc((size_t)cap * 16));
+        AUR_TRY(w.aov_a.alloc((size_t)cap * 12));
+        AUR_TRY(w.aov_n.alloc((size_t)cap * 12));
         w.capacity = cap;
     }
     AUR_TRY(w.counters.alloc(kCounterBytes));
@@ -447,8 +447,8 @@ Status ensure_workspace(aurora_ctx* c, uint32_t cap, int grid) {
 }
 
 // Paths per pass when the caller leaves spp_per_pass at 0.  A pass holds
-// 152 bytes of queues per path (two ping-pong queues, hit, radiance and the
-// two AOV planes), so 2^27 paths take 20 GB of the 288 GB of HBM: few, long
+// 144 bytes of queues per path (two ping-pong queues, hit, radiance and the
+// two AOV planes), so 2^27 paths take 19 GB of the 288 GB of HBM: few, long
 // passes amortise each launch's ramp and drain.
 constexpr uint64_t kPassPaths = 1ull << 27;
 

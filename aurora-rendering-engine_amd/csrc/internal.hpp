@@ -185,7 +185,7 @@ struct Workspace {
     DevBuf q_o[2], q_d[2], q_b[2];   // path queues (float4 each)
     DevBuf hit;                       // float2 per slot
     DevBuf L;                         // float4 per path id (radiance)
-    DevBuf aov_a, aov_n;              // float4 per path id
+    DevBuf aov_a, aov_n;              // 3 floats per path id
     DevBuf counters;                  // queue counts [0], [1]; pools from kPoolBase
     DevBuf spill;
     DevBuf recs;                      // SceneRec table of the current pass
